@@ -1,0 +1,251 @@
+"""ctypes front end of the CPU checker libraries (TEST INFRASTRUCTURE ONLY).
+
+Loads either the oracle restatement (lib/liborc_strict.so,
+lib/liborc_fast.so) or the reference harness (_ref/libxenon_ref*.so); both
+export the C API of oracle_api.h. Imported only by tests/ and by bench.py's
+CPU-baseline leg.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PATHS = {
+    "strict": os.path.join(HERE, "lib", "liborc_strict.so"),
+    "fast": os.path.join(HERE, "lib", "liborc_fast.so"),
+    "ref": os.path.join(HERE, "_ref", "libxenon_ref.so"),
+    "ref_fast": os.path.join(HERE, "_ref", "libxenon_ref_fast.so"),
+}
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_u32p = C.POINTER(C.c_uint32)
+_u64p = C.POINTER(C.c_uint64)
+_u8p = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+_SIGS = {
+    "orc_impl_name": (C.c_char_p, []),
+    "orc_scene_new": (_vp, []),
+    "orc_scene_free": (None, [_vp]),
+    "orc_scene_add_material": (C.c_int, [_vp, _fp]),
+    "orc_material_derived": (None, [_vp, C.c_int, _fp]),
+    "orc_scene_add_mesh": (C.c_int, [_vp, C.c_int, _fp, _fp, C.c_int, _ip, _ip]),
+    "orc_mesh_normals": (None, [_vp, C.c_int, _fp]),
+    "orc_scene_add_light": (C.c_int, [_vp, C.c_int, C.c_int, _fp, C.c_float]),
+    "orc_scene_add_dir_light": (C.c_int, [_vp, _fp, _fp, C.c_float]),
+    "orc_scene_set_sky": (None, [_vp, _fp, _fp, _fp, C.c_float, C.c_float]),
+    "orc_scene_add_instance": (C.c_int, [_vp, C.c_int, _fp, C.c_float, _fp, C.c_int]),
+    "orc_scene_build": (C.c_int, [_vp]),
+    "orc_scene_set_camera": (None, [_vp, _fp, _fp, _fp, C.c_float, C.c_float]),
+    "orc_camera_get": (None, [_vp, _fp]),
+    "orc_blas_node_count": (C.c_int, [_vp, C.c_int]),
+    "orc_blas_export": (None, [_vp, C.c_int, _vp, _u32p]),
+    "orc_tlas_node_count": (C.c_int, [_vp]),
+    "orc_instance_count": (C.c_int, [_vp]),
+    "orc_tlas_export": (None, [_vp, _vp, _u32p]),
+    "orc_instance_export": (None, [_vp, _fp, _ip, _ip]),
+    "orc_trace_closest": (None, [_vp, C.c_int, _fp, _fp, _ip, _u64p]),
+    "orc_trace_any": (None, [_vp, C.c_int, _fp, _u8p, _u64p]),
+    "orc_render": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp, _u64p]),
+    "orc_path": (None, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _fp]),
+    "orc_pcg_stream": (None, [C.c_uint64, C.c_uint64, C.c_int, _u32p]),
+    "orc_bsdf_sample": (None, [_fp, _fp, C.c_uint64, C.c_int, _fp]),
+    "orc_bsdf_eval": (None, [_fp, C.c_int, _fp, _fp, _fp]),
+}
+
+NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("child", "<u4"), ("count", "<u4")])
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float32).ctypes.data_as(_fp)
+
+
+def _arr(a, dt):
+    a = np.ascontiguousarray(a, dtype=dt)
+    return a
+
+
+class Lib:
+    """One loaded checker library."""
+
+    def __init__(self, kind: str = "strict"):
+        path = PATHS[kind]
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"checker library {path} is not built (run `make -C oracle all ref`)")
+        self.kind = kind
+        self.dll = C.CDLL(path)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(self.dll, name)
+            fn.restype = res
+            fn.argtypes = args
+        self.is_reference = self.dll.orc_impl_name().decode() == "reference"
+
+    # -- scene --------------------------------------------------------------
+    def load(self, desc) -> "OScene":
+        return OScene(self, desc)
+
+    def pcg(self, seed: int, seq: int, n: int) -> np.ndarray:
+        out = np.zeros(n, np.uint32)
+        self.dll.orc_pcg_stream(seed, seq, n, out.ctypes.data_as(_u32p))
+        return out
+
+    def bsdf_sample(self, params16, wo, seed: int, n: int) -> np.ndarray:
+        out = np.zeros((n, 9), np.float32)
+        p = _arr(params16, np.float32)
+        w = _arr(wo, np.float32)
+        self.dll.orc_bsdf_sample(p.ctypes.data_as(_fp), w.ctypes.data_as(_fp), seed, n, out.ctypes.data_as(_fp))
+        return out
+
+    def bsdf_eval(self, params16, wo, wi) -> np.ndarray:
+        wo = _arr(wo, np.float32).reshape(-1, 3)
+        wi = _arr(wi, np.float32).reshape(-1, 3)
+        out = np.zeros((wo.shape[0], 4), np.float32)
+        p = _arr(params16, np.float32)
+        self.dll.orc_bsdf_eval(p.ctypes.data_as(_fp), wo.shape[0], wo.ctypes.data_as(_fp), wi.ctypes.data_as(_fp),
+                               out.ctypes.data_as(_fp))
+        return out
+
+
+class OScene:
+    """A scene built inside one checker library from a SceneDesc."""
+
+    def __init__(self, lib: Lib, desc):
+        self.lib = lib
+        self.desc = desc
+        d = lib.dll
+        self.h = d.orc_scene_new()
+        for m in desc.materials:
+            p = m.params16()
+            d.orc_scene_add_material(self.h, p.ctypes.data_as(_fp))
+        self._keep = []
+        for m in desc.meshes:
+            pos = _arr(m.positions, np.float32)
+            idx = _arr(m.indices, np.int32)
+            mat = _arr(m.mat_ids, np.int32)
+            nrm = _arr(m.normals, np.float32) if m.normals is not None else None
+            d.orc_scene_add_mesh(self.h, pos.shape[0], pos.ctypes.data_as(_fp),
+                                 nrm.ctypes.data_as(_fp) if nrm is not None else None,
+                                 idx.shape[0], idx.ctypes.data_as(_ip), mat.ctypes.data_as(_ip))
+        for lt in desc.lights:
+            e = np.array(lt.emission, np.float32)
+            d.orc_scene_add_light(self.h, lt.mesh, lt.tri, e.ctypes.data_as(_fp), float(np.float32(lt.area)))
+        for s in desc.suns:
+            dd = np.array(s.direction, np.float32)
+            cc = np.array(s.color, np.float32)
+            d.orc_scene_add_dir_light(self.h, dd.ctypes.data_as(_fp), cc.ctypes.data_as(_fp), s.intensity)
+        sk = desc.sky
+        z, h_, g = (np.array(v, np.float32) for v in (sk.zenith, sk.horizon, sk.ground))
+        d.orc_scene_set_sky(self.h, z.ctypes.data_as(_fp), h_.ctypes.data_as(_fp), g.ctypes.data_as(_fp),
+                            sk.intensity, sk.sharpness)
+        for ins in desc.instances:
+            t = np.array(ins.translation, np.float32)
+            r = np.array(ins.rot_deg, np.float32)
+            d.orc_scene_add_instance(self.h, ins.mesh, t.ctypes.data_as(_fp), ins.scale, r.ctypes.data_as(_fp),
+                                     ins.instance_id)
+        cam = desc.camera
+        e, tg, up = (np.array(v, np.float32) for v in (cam.eye, cam.target, cam.up))
+        d.orc_scene_set_camera(self.h, e.ctypes.data_as(_fp), tg.ctypes.data_as(_fp), up.ctypes.data_as(_fp),
+                               cam.fov, desc.aspect)
+        rc = d.orc_scene_build(self.h)
+        if rc != 0:
+            raise RuntimeError(f"orc_scene_build failed ({rc})")
+
+    def __del__(self):
+        try:
+            self.lib.dll.orc_scene_free(self.h)
+        except Exception:
+            pass
+
+    def camera(self) -> np.ndarray:
+        out = np.zeros(12, np.float32)
+        self.lib.dll.orc_camera_get(self.h, out.ctypes.data_as(_fp))
+        return out
+
+    def blas(self, mesh: int):
+        n = self.lib.dll.orc_blas_node_count(self.h, mesh)
+        nodes = np.zeros(n, NODE_DTYPE)
+        slots = np.zeros(self.desc.meshes[mesh].num_tris, np.uint32)
+        self.lib.dll.orc_blas_export(self.h, mesh, nodes.ctypes.data_as(_vp), slots.ctypes.data_as(_u32p))
+        return nodes, slots
+
+    def tlas(self):
+        n = self.lib.dll.orc_tlas_node_count(self.h)
+        ni = self.lib.dll.orc_instance_count(self.h)
+        nodes = np.zeros(n, NODE_DTYPE)
+        order = np.zeros(ni, np.uint32)
+        self.lib.dll.orc_tlas_export(self.h, nodes.ctypes.data_as(_vp), order.ctypes.data_as(_u32p))
+        xf = np.zeros((ni, 24), np.float32)
+        ids = np.zeros(ni, np.int32)
+        meshes = np.zeros(ni, np.int32)
+        self.lib.dll.orc_instance_export(self.h, xf.ctypes.data_as(_fp), ids.ctypes.data_as(_ip),
+                                         meshes.ctypes.data_as(_ip))
+        return nodes, order, xf, ids, meshes
+
+    def normals(self, mesh: int) -> np.ndarray:
+        out = np.zeros((self.desc.meshes[mesh].positions.shape[0], 3), np.float32)
+        self.lib.dll.orc_mesh_normals(self.h, mesh, out.ctypes.data_as(_fp))
+        return out
+
+    def trace_closest(self, rays: np.ndarray, counters: bool = False):
+        rays = _arr(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hf = np.zeros((n, 12), np.float32)
+        hi = np.zeros((n, 4), np.int32)
+        cnt = np.zeros(4, np.uint64)
+        self.lib.dll.orc_trace_closest(self.h, n, rays.ctypes.data_as(_fp), hf.ctypes.data_as(_fp),
+                                       hi.ctypes.data_as(_ip), cnt.ctypes.data_as(_u64p) if counters else None)
+        return (hf, hi, cnt) if counters else (hf, hi)
+
+    def trace_any(self, rays: np.ndarray, counters: bool = False):
+        rays = _arr(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        occ = np.zeros(n, np.uint8)
+        cnt = np.zeros(4, np.uint64)
+        self.lib.dll.orc_trace_any(self.h, n, rays.ctypes.data_as(_fp), occ.ctypes.data_as(_u8p),
+                                   cnt.ctypes.data_as(_u64p) if counters else None)
+        return (occ, cnt) if counters else occ
+
+    def render_sum(self, spp_first: int, spp_count: int, nthreads: int = 0, width=None, height=None,
+                   accum=None):
+        """Restatement only: adds samples spp_first..spp_first+spp_count-1
+        into a (H*W, 3) float32 sum (pixel order); returns (sum, stats)."""
+        if self.lib.is_reference:
+            raise RuntimeError("the reference harness only renders averages from sample 1")
+        W = width or self.desc.width
+        H = height or self.desc.height
+        if accum is None:
+            accum = np.zeros((H * W, 3), np.float32)
+        stats = np.zeros(3, np.uint64)
+        nthreads = nthreads or (os.cpu_count() or 1)
+        rc = self.lib.dll.orc_render(self.h, W, H, spp_first, spp_count, self.desc.min_bounces,
+                                     self.desc.max_bounces, nthreads, accum.ctypes.data_as(_fp),
+                                     stats.ctypes.data_as(_u64p))
+        if rc != 0:
+            raise RuntimeError(f"orc_render failed ({rc})")
+        return accum, stats
+
+    def render_average(self, spp_count: int, nthreads: int = 0, width=None, height=None) -> np.ndarray:
+        """Average of samples 1..spp_count in pixel order, (H*W, 3), computed
+        as the reference finalises a frame: sum * (1/spp) in float32."""
+        W = width or self.desc.width
+        H = height or self.desc.height
+        if self.lib.is_reference:
+            out = np.zeros((H * W, 3), np.float32)
+            rc = self.lib.dll.orc_render(self.h, W, H, 1, spp_count, self.desc.min_bounces,
+                                         self.desc.max_bounces, nthreads, out.ctypes.data_as(_fp), None)
+            if rc != 0:
+                raise RuntimeError(f"reference render failed ({rc})")
+            return out
+        acc, _ = self.render_sum(1, spp_count, nthreads, W, H)
+        inv = np.float32(1.0) / np.float32(spp_count)
+        return acc * inv
+
+    def path(self, x, y, spp, width=None, height=None):
+        out = np.zeros(3, np.float32)
+        self.lib.dll.orc_path(self.h, width or self.desc.width, height or self.desc.height, x, y, spp,
+                              self.desc.min_bounces, self.desc.max_bounces, out.ctypes.data_as(_fp))
+        return out

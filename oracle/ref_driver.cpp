@@ -1,0 +1,292 @@
+// ref_driver.cpp — headless harness over the reference's own render core.
+//
+// TEST INFRASTRUCTURE ONLY. oracle/Makefile compiles this file together with
+// the reference's render-core sources, in place under /root/reference/src,
+// into oracle/_ref/libxenon_ref.so (strict IEEE flags) and
+// oracle/_ref/libxenon_ref_fast.so (the reference's Release flags). It
+// exports the C API of oracle/oracle_api.h using the reference's own types
+// (Scene, BLAS, TLAS, Camera, WavefrontRenderer, bsdf_*), so tests can
+// compare the oracle restatement against the reference byte for byte.
+// Nothing here is reference source; it only calls the reference's public
+// interfaces.
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "camera/camera.h"
+#include "camera/sampler.h"
+#include "material/bsdf.h"
+#include "material/material.h"
+#include "render/swapchain.h"
+#include "render/wavefront.h"
+#include "scene/scene.h"
+
+#include "oracle_api.h"
+
+namespace {
+struct RefScene {
+  xn::Scene scene;
+  xn::Camera camera;
+  struct PendingInst { int mesh; xn::Vec3 t; float s; xn::Vec3 rot; int id; };
+  std::vector<PendingInst> pending;
+  bool built = false;
+};
+RefScene* R(void* h) { return static_cast<RefScene*>(h); }
+
+xn::Material material_from(const float* p) {
+  xn::Material m;
+  m.baseColor = xn::Vec3(p[0], p[1], p[2]);
+  m.roughness = p[3];
+  m.metallic = p[4];
+  m.ior = p[5];
+  m.transmission = p[6];
+  m.subsurface = p[7];
+  m.subsurfaceColor = xn::Vec3(p[8], p[9], p[10]);
+  m.anisotropy = p[11];
+  m.emission = xn::Vec3(p[12], p[13], p[14]);
+  xn::material_prepare(m);
+  return m;
+}
+
+void export_nodes(const void* src, size_t n, void* dst) { std::memcpy(dst, src, n * 32); }
+}  // namespace
+
+extern "C" {
+
+const char* orc_impl_name(void) { return "reference"; }
+void* orc_scene_new(void) { return new RefScene(); }
+void orc_scene_free(void* h) { delete R(h); }
+
+int orc_scene_add_material(void* h, const float* p) {
+  R(h)->scene.materials.push_back(material_from(p));
+  return static_cast<int>(R(h)->scene.materials.size()) - 1;
+}
+
+void orc_material_derived(void* h, int i, float* out) {
+  const xn::Material& m = R(h)->scene.materials[i];
+  out[0] = m.alpha; out[1] = m.alpha_x; out[2] = m.alpha_y;
+  out[3] = m.F0.x; out[4] = m.F0.y; out[5] = m.F0.z;
+  out[6] = m.isConductor; out[7] = m.isTransmissive; out[8] = m.hasSubsurface; out[9] = m.isDelta;
+  out[10] = m.isEmissive;
+}
+
+// Fills a TriangleMesh the way load_obj does: vertices, triangles with
+// per-triangle material ids, and smooth normals when none are supplied.
+int orc_scene_add_mesh(void* h, int nv, const float* pos, const float* nrm, int nt, const int* idx, const int* mat) {
+  xn::TriangleMesh m;
+  for (int i = 0; i < nv; ++i) m.add_vertex(xn::Vec3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+  for (int t = 0; t < nt; ++t) m.add_triangle(idx[3 * t], idx[3 * t + 1], idx[3 * t + 2], mat[t]);
+  if (nrm) {
+    for (int i = 0; i < nv; ++i) { m.nx[i] = nrm[3 * i]; m.ny[i] = nrm[3 * i + 1]; m.nz[i] = nrm[3 * i + 2]; }
+  } else {
+    m.compute_smooth_normals();
+  }
+  R(h)->scene.meshes.push_back(std::move(m));
+  return static_cast<int>(R(h)->scene.meshes.size()) - 1;
+}
+
+void orc_mesh_normals(void* h, int mesh, float* out) {
+  const xn::TriangleMesh& m = R(h)->scene.meshes[mesh];
+  for (size_t i = 0; i < m.nx.size(); ++i) { out[3 * i] = m.nx[i]; out[3 * i + 1] = m.ny[i]; out[3 * i + 2] = m.nz[i]; }
+}
+
+int orc_scene_add_light(void* h, int mesh, int tri, const float* e, float area) {
+  xn::Light lt{};
+  lt.mesh_id = static_cast<uint32_t>(mesh);
+  lt.tri_idx = static_cast<uint32_t>(tri);
+  lt.emission = xn::Vec3(e[0], e[1], e[2]);
+  lt.area = area;
+  R(h)->scene.lights.push_back(lt);
+  return static_cast<int>(R(h)->scene.lights.size()) - 1;
+}
+
+// Normalised with the same Vec3 operations the scene parser applies.
+int orc_scene_add_dir_light(void* h, const float* d, const float* c, float intensity) {
+  xn::DirectionalLight dl;
+  dl.direction = xn::Vec3(d[0], d[1], d[2]);
+  dl.color = xn::Vec3(c[0], c[1], c[2]);
+  dl.intensity = intensity;
+  const float len = dl.direction.length();
+  if (len > 1e-6f) dl.direction = dl.direction / len;
+  else dl.direction = xn::Vec3(0.f, 1.f, 0.f);
+  R(h)->scene.dir_lights.push_back(dl);
+  return static_cast<int>(R(h)->scene.dir_lights.size()) - 1;
+}
+
+void orc_scene_set_sky(void* h, const float* z, const float* hz, const float* g, float intensity, float sharp) {
+  xn::Environment& s = R(h)->scene.sky;
+  s.zenith_color = xn::Vec3(z[0], z[1], z[2]);
+  s.horizon_color = xn::Vec3(hz[0], hz[1], hz[2]);
+  s.ground_color = xn::Vec3(g[0], g[1], g[2]);
+  s.intensity = intensity;
+  s.horizon_sharpness = sharp;
+}
+
+int orc_scene_add_instance(void* h, int mesh, const float* t, float s, const float* rot, int id) {
+  R(h)->pending.push_back({mesh, xn::Vec3(t[0], t[1], t[2]), s, xn::Vec3(rot[0], rot[1], rot[2]), id});
+  return static_cast<int>(R(h)->pending.size()) - 1;
+}
+
+// Without explicit instances this is Scene::build_acceleration itself; with
+// them, the same BLAS builds feed TLAS::build over the given instances.
+int orc_scene_build(void* h) {
+  RefScene& r = *R(h);
+  if (r.pending.empty()) {
+    r.scene.build_acceleration();
+  } else {
+    r.scene.blas_list.clear();
+    for (auto& m : r.scene.meshes) {
+      auto b = std::make_unique<xn::BLAS>();
+      b->build(m);
+      r.scene.blas_list.push_back(std::move(b));
+    }
+    std::vector<xn::Instance> inst;
+    for (const auto& p : r.pending) {
+      xn::Instance in;
+      in.geom = xn::GeomHandle::from_mesh(r.scene.blas_list[p.mesh].get());
+      in.xform = xn::AffineTransform::from_trs(p.t, p.s, p.rot);
+      in.instance_id = p.id;
+      in.rebuild_world_aabb();
+      inst.push_back(std::move(in));
+    }
+    r.scene.tlas.build(std::move(inst));
+  }
+  r.built = true;
+  return 0;
+}
+
+void orc_scene_set_camera(void* h, const float* e, const float* t, const float* u, float fov, float aspect) {
+  R(h)->camera.look_at(xn::Vec3(e[0], e[1], e[2]), xn::Vec3(t[0], t[1], t[2]), xn::Vec3(u[0], u[1], u[2]), fov,
+                       aspect);
+}
+
+void orc_camera_get(void* h, float* out) {
+  const xn::Camera& c = R(h)->camera;
+  const xn::Vec3 v[4] = {c.get_eye(), c.get_lower_left_corner(), c.get_horizontal(), c.get_vertical()};
+  for (int i = 0; i < 4; ++i) { out[3 * i] = v[i].x; out[3 * i + 1] = v[i].y; out[3 * i + 2] = v[i].z; }
+}
+
+int orc_blas_node_count(void* h, int mesh) { return static_cast<int>(R(h)->scene.blas_list[mesh]->nodes().size()); }
+void orc_blas_export(void* h, int mesh, void* nodes, uint32_t* slots) {
+  const xn::BLAS& b = *R(h)->scene.blas_list[mesh];
+  export_nodes(b.nodes().data(), b.nodes().size(), nodes);
+  std::memcpy(slots, b.prims().data(), b.prims().size() * sizeof(uint32_t));
+}
+int orc_tlas_node_count(void* h) { return static_cast<int>(R(h)->scene.tlas.nodes().size()); }
+int orc_instance_count(void* h) { return static_cast<int>(R(h)->scene.tlas.instances().size()); }
+void orc_tlas_export(void* h, void* nodes, uint32_t* order) {
+  const xn::TLAS& t = R(h)->scene.tlas;
+  export_nodes(t.nodes().data(), t.nodes().size(), nodes);
+  std::memcpy(order, t.inst_indices().data(), t.inst_indices().size() * sizeof(uint32_t));
+}
+void orc_instance_export(void* h, float* xf24, int* ids, int* meshes) {
+  const RefScene& r = *R(h);
+  const auto& inst = r.scene.tlas.instances();
+  for (size_t i = 0; i < inst.size(); ++i) {
+    for (int c = 0; c < 4; ++c) {
+      const xn::Vec3 a = inst[i].xform.local_to_world.cols[c];
+      const xn::Vec3 b = inst[i].xform.world_to_local.cols[c];
+      xf24[24 * i + 3 * c] = a.x; xf24[24 * i + 3 * c + 1] = a.y; xf24[24 * i + 3 * c + 2] = a.z;
+      xf24[24 * i + 12 + 3 * c] = b.x; xf24[24 * i + 12 + 3 * c + 1] = b.y; xf24[24 * i + 12 + 3 * c + 2] = b.z;
+    }
+    ids[i] = inst[i].instance_id;
+    int mesh = -1;
+    for (size_t k = 0; k < r.scene.blas_list.size(); ++k)
+      if (inst[i].geom.mesh == r.scene.blas_list[k].get()) mesh = static_cast<int>(k);
+    meshes[i] = mesh;
+  }
+}
+
+static xn::Ray load_ray(const float* r) {
+  xn::Ray ray;
+  ray.origin = xn::Vec3(r[0], r[1], r[2]);
+  ray.dir = xn::Vec3(r[3], r[4], r[5]);
+  ray.tmin = r[6];
+  ray.tmax = r[7];
+  return ray;
+}
+
+void orc_trace_closest(void* h, int n, const float* rays, float* hf, int* hi, uint64_t* counters) {
+  const xn::Scene& sc = R(h)->scene;
+  for (int i = 0; i < n; ++i) {
+    xn::HitRecord rec;
+    const bool ok = sc.intersect(load_ray(rays + 8 * i), rec);
+    float* o = hf + 12 * i;
+    int* oi = hi + 4 * i;
+    if (!ok) {
+      for (int k = 0; k < 12; ++k) o[k] = 0.f;
+      o[0] = xn::kInfinity;
+      oi[0] = -1; oi[1] = -1; oi[2] = -1; oi[3] = 0;
+      continue;
+    }
+    const float v[12] = {rec.t, rec.u, rec.v, rec.pos.x, rec.pos.y, rec.pos.z, rec.normal.x, rec.normal.y,
+                         rec.normal.z, rec.geo_normal.x, rec.geo_normal.y, rec.geo_normal.z};
+    std::memcpy(o, v, sizeof(v));
+    oi[0] = rec.instance_id; oi[1] = rec.prim_id; oi[2] = rec.mat_id; oi[3] = rec.front_face ? 1 : 0;
+  }
+  if (counters) for (int k = 0; k < 4; ++k) counters[k] = 0;
+}
+
+void orc_trace_any(void* h, int n, const float* rays, uint8_t* occ, uint64_t* counters) {
+  const xn::Scene& sc = R(h)->scene;
+  for (int i = 0; i < n; ++i) occ[i] = sc.intersects(load_ray(rays + 8 * i)) ? 1 : 0;
+  if (counters) for (int k = 0; k < 4; ++k) counters[k] = 0;
+}
+
+// The reference renderer always starts at sample 1 and keeps its sums
+// private; it only exposes the running average through the swapchain, with
+// rows flipped. This entry therefore supports spp_first == 1 and returns
+// the AVERAGE (not the sum) in pixel order. stats are not available.
+int orc_render(void* h, int W, int H, int spp_first, int spp_count, int minb, int maxb, int nthreads, float* out,
+               uint64_t* stats) {
+  (void)nthreads;
+  RefScene& r = *R(h);
+  if (!r.built || spp_first != 1 || spp_count < 1) return -2;
+  xn::WavefrontRenderer renderer(W, H);
+  renderer.set_bounces(minb, maxb);
+  xn::TripleSwapchain chain(W, H);
+  for (int i = 0; i < spp_count; ++i) renderer.render_frame_tiled(r.scene, r.camera, chain);
+  const float* img = chain.get_read_buffer();
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x)
+      for (int c = 0; c < 3; ++c)
+        out[3 * (static_cast<size_t>(y) * W + x) + c] = img[3 * (static_cast<size_t>(H - 1 - y) * W + x) + c];
+  if (stats) { stats[0] = 0; stats[1] = 0; stats[2] = 0; }
+  return 0;
+}
+
+void orc_path(void*, int, int, int, int, int, int, int, float* out3) {
+  out3[0] = out3[1] = out3[2] = __builtin_nanf("");
+}
+
+void orc_pcg_stream(uint64_t seed, uint64_t seq, int n, uint32_t* out) {
+  xn::PCGState p = xn::seed_pcg(seed, seq);
+  for (int i = 0; i < n; ++i) out[i] = p.next_uint();
+}
+
+void orc_bsdf_sample(const float* p, const float* wo, uint64_t seed, int n, float* out) {
+  const xn::Material m = material_from(p);
+  xn::PCGState rng = xn::seed_pcg(seed);
+  const xn::Vec3 w(wo[0], wo[1], wo[2]);
+  for (int i = 0; i < n; ++i) {
+    xn::BSDFSample s;
+    const bool ok = xn::bsdf_sample(w, m, rng, s);
+    float* o = out + 9 * i;
+    o[0] = s.wi.x; o[1] = s.wi.y; o[2] = s.wi.z; o[3] = s.f.x; o[4] = s.f.y; o[5] = s.f.z;
+    o[6] = s.pdf; o[7] = s.is_delta ? 1.f : 0.f; o[8] = ok ? 1.f : 0.f;
+  }
+}
+
+void orc_bsdf_eval(const float* p, int n, const float* wo, const float* wi, float* out) {
+  const xn::Material m = material_from(p);
+  for (int i = 0; i < n; ++i) {
+    const xn::Vec3 a(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+    const xn::Vec3 b(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+    const xn::Vec3 f = xn::bsdf_eval_for_nee(a, b, m);
+    out[4 * i] = f.x; out[4 * i + 1] = f.y; out[4 * i + 2] = f.z;
+    out[4 * i + 3] = xn::bsdf_pdf_for_nee(a, b, m);
+  }
+}
+
+}  // extern "C"
