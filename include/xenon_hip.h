@@ -1,0 +1,233 @@
+/* xenon_hip.h — C ABI of the MI355X (gfx950) wavefront path-tracing backend.
+ *
+ * libxenon_hip.so is the drop-in for the reference's GPU backend contract,
+ * CudaRenderer (src/cuda/cuda_renderer.cuh:14-55):
+ *
+ *   CudaRenderer(w, h, max_bounces, min_bounces)  -> xnh_create
+ *   CudaRenderer::upload_scene(const Scene&)      -> xnh_upload_scene
+ *   CudaRenderer::render_frame(scene, cam, chain) -> xnh_render (spp_count 1)
+ *   CudaRenderer::spp()                           -> xnh_spp
+ *   ~CudaRenderer()                               -> xnh_destroy
+ *   CUDA_CHECK abort (cuda/cuda_utils.cuh:12-20)  -> negative return codes
+ *                                                    + xnh_last_error
+ *
+ * Numerics follow the reference's CPU oracle, render/cpu_tiled.cpp:13-237
+ * (same per-pixel seeds, RNG draw order, BSDF, NEE/MIS and Russian
+ * roulette), not the reference's CUDA kernels.
+ *
+ * Conventions: every int-returning call returns XNH_OK (0) or a negative
+ * XNH_E_* code and never aborts; the library copies what it needs and keeps
+ * no caller pointer; one context is used by one thread at a time. Images
+ * are W*H*3 float32 in pixel order (index y*W+x, y = 0 the bottom row, as
+ * the reference's accumulation buffer); the reference's swapchain writer
+ * flips rows (cpu_tiled.cpp:228-229) and a wrapper does the same.
+ */
+#ifndef XENON_HIP_H
+#define XENON_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XNH_OK 0
+#define XNH_E_INVALID (-1)     /* bad argument */
+#define XNH_E_HIP (-2)         /* HIP runtime error (message in xnh_last_error) */
+#define XNH_E_NO_SCENE (-3)    /* render/trace before xnh_upload_scene */
+#define XNH_E_OVERFLOW (-4)    /* a traversal stack exceeded its limit */
+#define XNH_E_NO_DEVICE (-5)   /* no usable gfx950 device */
+#define XNH_E_ALLOC (-6)       /* device or host allocation failed */
+
+typedef struct xnh_ctx xnh_ctx;
+typedef struct xnh_scene xnh_scene;
+
+/* Renderer configuration. Zero fields take defaults. */
+typedef struct {
+  int32_t width, height;
+  int32_t max_bounces;       /* reference default 8 (scene/scene_file.h:16) */
+  int32_t min_bounces;       /* Russian roulette from this depth, default 3 */
+  int32_t spp_per_launch;    /* samples per pixel in flight per wavefront batch (0: auto) */
+  int32_t block_size;        /* work-group size of the traversal kernels (0: auto) */
+  int32_t lds_stack;         /* traversal stack entries held in LDS per lane (0: auto) */
+  int32_t flags;             /* reserved, 0 */
+} xnh_config;
+
+/* Pinhole camera as the reference's Camera exposes it for GPU upload
+ * (camera/camera.h:40-43). */
+typedef struct {
+  float eye[3], lower_left[3], horizontal[3], vertical[3];
+} xnh_camera;
+
+/* Binary-identical to the reference's BLASNode and TLASNode
+ * (geometry/blas.h:21-28, geometry/tlas.h:91-96). */
+typedef struct {
+  float lo[3], hi[3];
+  uint32_t child;            /* interior: left child (right = left+1); leaf: first slot */
+  uint32_t count;            /* 0: interior; >0: leaf of `count` slots */
+} xnh_bvh_node;
+
+/* One triangle mesh with its BLAS, as TriangleMesh (geometry/mesh.h:21-31)
+ * and BLAS::nodes()/prims() (geometry/blas.h:59-60) hold them. */
+typedef struct {
+  int32_t num_vertices, num_triangles;
+  const float *vx, *vy, *vz;           /* positions, SoA */
+  const float *nx, *ny, *nz;           /* vertex normals, SoA (may be NULL: flat) */
+  const int32_t* indices;              /* 3 per triangle */
+  const int32_t* mat_ids;              /* 1 per triangle */
+  int32_t num_nodes;
+  const xnh_bvh_node* nodes;
+  const uint32_t* prims;               /* leaf slot -> triangle index */
+} xnh_mesh;
+
+/* Instance (geometry/tlas.h:76-87): column-major 3x4 matrices as in
+ * AffineTransform (geometry/transform.h:24-62). */
+typedef struct {
+  int32_t mesh;
+  int32_t instance_id;
+  float local_to_world[12];
+  float world_to_local[12];
+} xnh_instance;
+
+#define XNH_MAT_CONDUCTOR 1u
+#define XNH_MAT_TRANSMISSIVE 2u
+#define XNH_MAT_SUBSURFACE 4u
+#define XNH_MAT_DELTA 8u
+#define XNH_MAT_EMISSIVE 16u
+
+/* Prepared material (material/material.h:19-49 after material_prepare). */
+typedef struct {
+  float base_color[3];
+  float metallic, ior, alpha, alpha_x, alpha_y;
+  float f0[3];
+  float subsurface;
+  float subsurface_color[3];
+  uint32_t flags;
+} xnh_material;
+
+/* Area light (scene/scene.h:19-24): triangle tri_idx of mesh mesh_id. */
+typedef struct {
+  uint32_t mesh_id, tri_idx;
+  float emission[3];
+  float area;
+} xnh_light;
+
+/* Directional light (scene/scene.h:43-49); emission = color * intensity,
+ * direction toward the sun, unit length. */
+typedef struct {
+  float direction[3];
+  float emission[3];
+} xnh_dir_light;
+
+/* Gradient sky (scene/environment.h:26-51); intensity 0 disables it. */
+typedef struct {
+  float zenith[3], horizon[3], ground[3];
+  float sharpness, intensity;
+} xnh_sky;
+
+/* Everything upload_scene reads from the reference's Scene
+ * (scene/scene.h:52-75) after build_acceleration. */
+typedef struct {
+  int32_t num_meshes;
+  const xnh_mesh* meshes;
+  int32_t num_instances;
+  const xnh_instance* instances;
+  int32_t num_tlas_nodes;
+  const xnh_bvh_node* tlas_nodes;
+  const uint32_t* tlas_inst_indices;   /* TLAS::inst_indices() */
+  int32_t num_materials;
+  const xnh_material* materials;
+  int32_t num_lights;
+  const xnh_light* lights;
+  int32_t num_dir_lights;
+  const xnh_dir_light* dir_lights;
+  xnh_sky sky;
+} xnh_flat_scene;
+
+typedef struct {
+  float org[3], dir[3];
+  float tmin, tmax;
+} xnh_ray;
+
+/* Closest hit as the reference's HitRecord after TLAS fix-up
+ * (math/ray.h:47-59, geometry/tlas.cpp:156-168). Miss: t = +inf, ids -1. */
+typedef struct {
+  float t, u, v;
+  int32_t instance_id, prim_id, mat_id;
+  float pos[3], normal[3], geo_normal[3];
+  int32_t front_face;
+} xnh_hit;
+
+/* Counters of the last xnh_render / xnh_trace_* call. */
+typedef struct {
+  uint64_t paths;            /* camera paths started */
+  uint64_t closest_rays;     /* closest-hit rays traced */
+  uint64_t shadow_rays;      /* shadow rays traced */
+  uint64_t node_tests;       /* BVH boxes tested (BLAS pair = 2, TLAS pop = 1) */
+  uint64_t interior;         /* interior nodes expanded */
+  uint64_t tri_tests;        /* ray-triangle tests */
+  uint64_t inst_visits;      /* instance ray transforms */
+  uint64_t nonfinite;        /* non-finite path radiances (kept, as the reference does) */
+  double render_ms;          /* device time of the call (HIP events) */
+  double closest_ms;         /* device time inside the closest-hit kernels */
+  double shadow_ms;          /* device time inside the shadow kernels */
+  double shade_ms;           /* device time inside the shading kernels */
+  int32_t closest_launches;  /* closest-hit kernel launches in the call */
+  int32_t stack_overflows;   /* lanes whose traversal stack exceeded its limit */
+} xnh_stats;
+
+/* ---- renderer (CudaRenderer contract) ---------------------------------- */
+int xnh_create(const xnh_config* cfg, xnh_ctx** out);
+void xnh_destroy(xnh_ctx* ctx);
+const char* xnh_last_error(const xnh_ctx* ctx);
+int xnh_upload_scene(xnh_ctx* ctx, const xnh_flat_scene* scene);
+/* Adds samples spp_first .. spp_first+spp_count-1 (1-based sample indices,
+ * the value the reference's spp_ holds while rendering that frame) to the
+ * device accumulator. rgb_out (optional, host, W*H*3) receives the running
+ * average over all samples added since the last reset. */
+int xnh_render(xnh_ctx* ctx, const xnh_camera* cam, int32_t spp_first, int32_t spp_count, float* rgb_out);
+/* Copies the raw per-pixel sums (W*H*3) and returns the sample count. */
+int xnh_read_accum(xnh_ctx* ctx, float* sum_out, int32_t* spp_out);
+int xnh_reset(xnh_ctx* ctx);
+int xnh_spp(const xnh_ctx* ctx);
+int xnh_get_stats(const xnh_ctx* ctx, xnh_stats* out);
+/* Turns per-kernel HIP-event timing on (1) or off (0, default). */
+int xnh_set_profiling(xnh_ctx* ctx, int32_t on);
+
+/* ---- ray-batch parity API ----------------------------------------------- */
+/* Scene::intersect / Scene::intersects (scene/scene.h:126-131) for a batch
+ * of host rays; results copied back to host. */
+int xnh_trace_closest(xnh_ctx* ctx, const xnh_ray* rays, int32_t n, xnh_hit* hits);
+int xnh_trace_any(xnh_ctx* ctx, const xnh_ray* rays, int32_t n, uint8_t* occluded);
+
+/* ---- host scene builder -------------------------------------------------
+ * For callers without a reference Scene: the reference's scene model and
+ * Scene::build_acceleration (scene/scene.h:79-121) with the reference's
+ * binned-SAH BLAS/TLAS builders (geometry/blas.cpp:97-203,
+ * geometry/tlas.cpp:40-145), producing the same trees node for node.
+ * Material parameters: 16 floats (base rgb, roughness, metallic, ior,
+ * transmission, subsurface, subsurface rgb, anisotropy, emission rgb, pad);
+ * prepared as material_prepare does (material/material.h:55-75). */
+xnh_scene* xnh_scene_new(void);
+void xnh_scene_free(xnh_scene* s);
+int xnh_scene_add_material(xnh_scene* s, const float* params16);
+/* pos/nrm interleaved xyz; nrm NULL -> area-weighted smooth normals
+ * (geometry/mesh.cpp:12-34). */
+int xnh_scene_add_mesh(xnh_scene* s, int32_t nv, const float* pos, const float* nrm, int32_t nt,
+                       const int32_t* indices, const int32_t* mat_ids);
+int xnh_scene_add_light(xnh_scene* s, int32_t mesh, int32_t tri, const float* emission3, float area);
+int xnh_scene_add_dir_light(xnh_scene* s, const float* dir3, const float* color3, float intensity);
+int xnh_scene_set_sky(xnh_scene* s, const float* zenith3, const float* horizon3, const float* ground3,
+                      float intensity, float sharpness);
+/* TRS instance (geometry/transform.h:105-148); none -> identity per mesh. */
+int xnh_scene_add_instance(xnh_scene* s, int32_t mesh, const float* t3, float scale, const float* rot_deg3,
+                           int32_t instance_id);
+int xnh_scene_build(xnh_scene* s, int32_t nthreads);
+/* View of the built scene; valid until the scene is modified or freed. */
+int xnh_scene_flat(const xnh_scene* s, xnh_flat_scene* out);
+/* Camera::look_at (camera/camera.h:15-32). */
+int xnh_camera_look_at(const float* eye3, const float* target3, const float* up3, float fov_deg, float aspect,
+                       xnh_camera* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -60,6 +60,16 @@ _SIGS = {
 NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("child", "<u4"), ("count", "<u4")])
 
 
+def default_threads() -> int:
+    """Worker threads for CPU renders: the CPUs this process may run on,
+    capped at 16 (the CPU share of one GPU on a shared host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def _f(a):
     return np.ascontiguousarray(a, dtype=np.float32).ctypes.data_as(_fp)
 
@@ -220,7 +230,7 @@ class OScene:
         if accum is None:
             accum = np.zeros((H * W, 3), np.float32)
         stats = np.zeros(3, np.uint64)
-        nthreads = nthreads or (os.cpu_count() or 1)
+        nthreads = nthreads or default_threads()
         rc = self.lib.dll.orc_render(self.h, W, H, spp_first, spp_count, self.desc.min_bounces,
                                      self.desc.max_bounces, nthreads, accum.ctypes.data_as(_fp),
                                      stats.ctypes.data_as(_u64p))

@@ -1,0 +1,107 @@
+"""Seeded ray batches for traversal parity tests (rows: o.xyz, d.xyz, tmin, tmax)."""
+from __future__ import annotations
+
+import numpy as np
+
+F32 = np.float32
+
+
+def _unit(v):
+    v = v.astype(np.float64)
+    return (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(F32)
+
+
+def scene_bounds(desc):
+    lo = np.full(3, np.inf)
+    hi = np.full(3, -np.inf)
+    if desc.instances:
+        for ins in desc.instances:
+            P = desc.meshes[ins.mesh].positions.astype(np.float64)
+            r = np.abs(P).max() * ins.scale * 1.8 + 1e-3
+            c = np.array(ins.translation)
+            lo = np.minimum(lo, c - r)
+            hi = np.maximum(hi, c + r)
+    else:
+        for m in desc.meshes:
+            lo = np.minimum(lo, m.positions.min(0))
+            hi = np.maximum(hi, m.positions.max(0))
+    return lo, hi
+
+
+def camera_rays(cam12, n, rng):
+    eye, llc, hz, vt = cam12[0:3], cam12[3:6], cam12[6:9], cam12[9:12]
+    s = rng.random(n).astype(F32)
+    t = rng.random(n).astype(F32)
+    d = llc[None] + s[:, None] * hz[None] + t[:, None] * vt[None] - eye[None]
+    d = d / np.sqrt((d * d).sum(1, keepdims=True)).astype(F32)
+    o = np.repeat(eye[None], n, 0)
+    return np.concatenate([o, d.astype(F32), np.full((n, 1), 1e-4, F32), np.full((n, 1), np.inf, F32)], 1)
+
+
+def random_rays(desc, n, rng, tmax=np.inf):
+    lo, hi = scene_bounds(desc)
+    ext = hi - lo
+    o = (lo + (rng.random((n, 3)) * 1.2 - 0.1) * ext).astype(F32)
+    d = _unit(rng.normal(size=(n, 3)))
+    return np.concatenate([o, d, np.full((n, 1), 1e-4, F32), np.full((n, 1), tmax, F32)], 1)
+
+
+def vertex_rays(desc, n, rng):
+    """Rays aimed exactly at mesh vertices and edge midpoints: shared edges and
+    corners, where equal-t ties between neighbouring triangles occur."""
+    lo, hi = scene_bounds(desc)
+    mi = rng.integers(0, len(desc.meshes), n)
+    out = np.zeros((n, 8), F32)
+    for k in range(n):
+        m = desc.meshes[mi[k]]
+        t = rng.integers(0, m.num_tris)
+        a, b = m.indices[t][rng.permutation(3)[:2]]
+        p = m.positions[a].astype(np.float64)
+        if rng.random() < 0.5:
+            p = 0.5 * (p + m.positions[b])
+        if desc.instances:   # aim at the identity instance when there is one
+            pass
+        o = lo + rng.random(3) * (hi - lo)
+        d = p - o
+        out[k, :3] = o
+        out[k, 3:6] = d / np.linalg.norm(d)
+        out[k, 6] = 1e-4
+        out[k, 7] = np.inf
+    return out
+
+
+def bounce_rays(hf, hi, n, rng):
+    """Rays leaving closest-hit points along random directions, offset as the
+    integrator offsets bounce rays (1e-4 along the geometric normal)."""
+    ok = np.where(hi[:, 0] >= 0)[0]
+    if len(ok) == 0:
+        return np.zeros((0, 8), F32)
+    pick = ok[rng.integers(0, len(ok), n)]
+    pos = hf[pick, 3:6]
+    gn = hf[pick, 9:12]
+    d = _unit(rng.normal(size=(n, 3)))
+    flip = (gn * d).sum(1) < 0
+    g = gn.copy()
+    g[flip] = -g[flip]
+    o = (pos + g * F32(1e-4)).astype(F32)
+    return np.concatenate([o, d, np.full((n, 1), 1e-4, F32), np.full((n, 1), np.inf, F32)], 1)
+
+
+def shadow_rays(rays, hf, hi, rng):
+    """Segments between two points of a ray batch with finite tmax."""
+    n = rays.shape[0]
+    o = rays[:, :3]
+    target = o + rays[:, 3:6] * F32(2.0)
+    has = hi[:, 0] >= 0
+    target[has] = hf[has, 3:6]
+    d = target - o
+    dist = np.sqrt((d * d).sum(1)).astype(F32)
+    dist[dist == 0] = 1
+    d = (d / dist[:, None]).astype(F32)
+    return np.concatenate([o, d, np.full((n, 1), 5e-4, F32), (dist - F32(5e-4))[:, None]], 1).astype(F32)
+
+
+def standard_batch(desc, cam12, n_each=4096, seed=0):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([camera_rays(cam12, n_each, rng), random_rays(desc, n_each, rng),
+                           vertex_rays(desc, n_each // 4, rng)]).astype(F32)

@@ -1,0 +1,89 @@
+"""Ray-batch traversal parity: HIP closest/any hit vs the CPU oracle on
+identical rays. Integer results (instance, triangle, material ids, facing,
+occlusion) and the traversal work counters must be bit-identical; float
+results are compared as raw bits and must be identical too, apart from a
+grazing-edge fraction that the tolerance below bounds at 1e-5."""
+import numpy as np
+import pytest
+
+from raysets import bounce_rays, shadow_rays, standard_batch
+
+pytestmark = pytest.mark.gpu
+
+GRAZING_MAX = 1e-5     # fraction of rays allowed to pick a different tie winner
+T_RTOL = 1e-5          # hit-t relative error bound on those rays
+
+
+def _scenes():
+    from xenon_amd import scenes
+    return {
+        "cornell": lambda: scenes.cornell(64, 64),
+        "big_mesh": lambda: scenes.big_mesh(width=64, height=36, level=6),
+        "instancing": lambda: scenes.instancing(width=64, height=36, n_inst=512, tri_range=(2000, 20000), n_blas=10,
+                                                n_emit=64),
+    }
+
+
+@pytest.fixture(scope="module", params=["cornell", "big_mesh", "instancing"])
+def pair(request, oracle):
+    from xenon_amd import HipRenderer, HipScene
+    print(f"[trace parity] scene {request.param}", flush=True)
+    desc = _scenes()[request.param]()
+    hs = HipScene(desc)
+    r = HipRenderer(desc.width, desc.height, desc.max_bounces, desc.min_bounces)
+    r.upload_scene(hs)
+    yield desc, oracle.load(desc), r
+    r.close()
+
+
+def _compare_closest(hf, hi, hits):
+    ids_dev = np.stack([hits["instance_id"], hits["prim_id"], hits["mat_id"], hits["front_face"]], 1)
+    f_dev = np.concatenate([hits["t"][:, None], hits["u"][:, None], hits["v"][:, None], hits["pos"], hits["normal"],
+                            hits["geo_normal"]], 1).astype(np.float32)
+    id_bad = np.any(ids_dev != hi, axis=1)
+    f_bad = np.any(f_dev.view(np.uint32) != hf.view(np.uint32), axis=1)
+    return id_bad, f_bad, f_dev
+
+
+def test_closest_bit_exact(pair):
+    desc, osc, r = pair
+    rays = standard_batch(desc, osc.camera(), 4096, seed=1)
+    hf, hi, cnt = osc.trace_closest(rays, counters=True)
+    rays2 = bounce_rays(hf, hi, 4096, np.random.default_rng(2))
+    hf2, hi2, cnt2 = osc.trace_closest(rays2, counters=True)
+    for rr, (f, i, c) in ((rays, (hf, hi, cnt)), (rays2, (hf2, hi2, cnt2))):
+        hits = r.trace_closest(rr)
+        st = r.stats()
+        id_bad, f_bad, f_dev = _compare_closest(f, i, hits)
+        frac = float(np.mean(id_bad | f_bad))
+        print(f"  {desc.name}: {len(rr)} rays, id mismatches {id_bad.sum()}, float-bit mismatches {f_bad.sum()}, "
+              f"hits {np.sum(i[:, 0] >= 0)}", flush=True)
+        assert frac <= GRAZING_MAX, f"{frac} of rays differ"
+        both = (i[:, 0] >= 0) & ~id_bad
+        assert np.allclose(f_dev[both, 0], f[both, 0], rtol=T_RTOL, atol=0)
+        if frac == 0.0:
+            # identical rays through identical trees: identical work
+            assert [st["node_tests"], st["interior"], st["tri_tests"], st["inst_visits"]] == [int(x) for x in c]
+
+
+def test_any_bit_exact(pair):
+    desc, osc, r = pair
+    rays = standard_batch(desc, osc.camera(), 4096, seed=3)
+    hf, hi = osc.trace_closest(rays)
+    srays = np.concatenate([shadow_rays(rays, hf, hi, np.random.default_rng(4)), rays])
+    occ_ref, cnt = osc.trace_any(srays, counters=True)
+    occ = r.trace_any(srays)
+    st = r.stats()
+    bad = int(np.sum(occ != occ_ref))
+    print(f"  {desc.name}: {len(srays)} shadow rays, occluded {int(occ_ref.sum())}, mismatches {bad}", flush=True)
+    assert bad / len(srays) <= GRAZING_MAX
+    if bad == 0:
+        assert [st["node_tests"], st["interior"], st["tri_tests"], st["inst_visits"]] == [int(x) for x in cnt]
+
+
+def test_empty_batch_and_errors(pair):
+    desc, osc, r = pair
+    hits = r.trace_closest(np.zeros((0, 8), np.float32))
+    assert hits.shape == (0,)
+    occ = r.trace_any(np.zeros((0, 8), np.float32))
+    assert occ.shape == (0,)

@@ -1,0 +1,28 @@
+// launch.h — host-callable launchers of the device kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../common/dev_types.h"
+
+namespace xnh {
+
+// Persistent grid for the traversal kernels: CUs x resident blocks.
+int trace_grid(int block, int lds_depth);
+int num_cus();
+
+hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, const DevCamera& cam, int W, int H, int spp_first,
+                         int nsamples, int ncursors);
+hipError_t launch_closest(hipStream_t s, const DevScene& S, const WaveBuffers& B, int qi, int cursor, int grid,
+                          int block, int lds_depth);
+hipError_t launch_shade(hipStream_t s, const DevScene& S, const WaveBuffers& B, int qi, int depth, int min_b,
+                        int max_b, int grid);
+hipError_t launch_shadow(hipStream_t s, const DevScene& S, const WaveBuffers& B, int cursor, int grid, int block,
+                         int lds_depth);
+hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int npix, int nsamples);
+hipError_t launch_average(hipStream_t s, const float4* accum, float* out, int npix, float inv_spp);
+hipError_t launch_api_closest(hipStream_t s, const DevScene& S, const float4* rays, float* hits, int n,
+                              uint32_t* spill, int lds_depth, unsigned long long* stats, uint32_t* overflow_ctr);
+hipError_t launch_api_any(hipStream_t s, const DevScene& S, const float4* rays, uint8_t* occ, int n, uint32_t* spill,
+                          int lds_depth, unsigned long long* stats, uint32_t* overflow_ctr);
+
+}  // namespace xnh

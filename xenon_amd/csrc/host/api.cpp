@@ -1,0 +1,788 @@
+// api.cpp — the C ABI of libxenon_hip.so (include/xenon_hip.h).
+//
+// Host half of the backend: device selection, buffer ownership, scene
+// flattening into the HBM layout of common/dev_types.h, and the bounce loop
+// that enqueues the wavefront kernels on one HIP stream. The loop never
+// reads a count back: every kernel finds its work count in device memory,
+// so a call enqueues 2 + 3*max_bounces launches per batch and synchronises
+// once at the end.
+//
+// The reference's CUDA backend aborts on any CUDA error
+// (cuda/cuda_utils.cuh:12-20) and reads counters back ~6 times per bounce
+// (cuda/cuda_renderer.cu:516-573); neither is carried over.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../common/dev_types.h"
+#include "../device/launch.h"
+#include "scene.h"
+#include "xenon_hip.h"
+
+using namespace xnh;
+
+struct xnh_scene {
+  HostScene s;
+};
+
+struct xnh_ctx {
+  xnh_config cfg{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  std::vector<hipEvent_t> prof_events;
+  bool profiling = false;
+  std::vector<void*> scene_allocs;
+  DevScene S{};
+  bool has_scene = false;
+  WaveBuffers B{};
+  std::vector<void*> wave_allocs;
+  int nsamples = 1;        // samples per pixel per batch
+  int block = 256;
+  int lds_depth = 32;
+  int grid_trace = 0;
+  int grid_shade = 0;
+  float* d_out = nullptr;
+  int spp_total = 0;
+  xnh_stats stats{};
+};
+
+namespace {
+
+int fail(xnh_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_OR_FAIL(ctx, expr)                                                              \
+  do {                                                                                      \
+    const hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((ctx), XNH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+template <class T>
+int dev_alloc(xnh_ctx* c, std::vector<void*>& owner, T** out, size_t count) {
+  *out = nullptr;
+  if (count == 0) count = 1;
+  void* p = nullptr;
+  const hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) return fail(c, XNH_E_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+  owner.push_back(p);
+  *out = static_cast<T*>(p);
+  return XNH_OK;
+}
+
+template <class T>
+int dev_upload(xnh_ctx* c, std::vector<void*>& owner, T** out, const std::vector<T>& host) {
+  int rc = dev_alloc(c, owner, out, host.size());
+  if (rc != XNH_OK) return rc;
+  if (!host.empty()) {
+    const hipError_t e = hipMemcpy(*out, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  }
+  return XNH_OK;
+}
+
+void free_all(std::vector<void*>& v) {
+  for (void* p : v) hipFree(p);
+  v.clear();
+}
+
+float4 f4(float x, float y, float z, float w) {
+  float4 r;
+  r.x = x; r.y = y; r.z = z; r.w = w;
+  return r;
+}
+float bits(uint32_t u) {
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+float ibits(int32_t i) {
+  float f;
+  std::memcpy(&f, &i, 4);
+  return f;
+}
+
+// Geometric normal of a light triangle, mesh.h:64-68 (host IEEE order).
+void face_normal(const float p0[3], const float p1[3], const float p2[3], float out[3]) {
+  const float a[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+  const float b[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+  const float c[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+  const float len = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+  const float r = 1.f / len;
+  out[0] = c[0] * r;
+  out[1] = c[1] * r;
+  out[2] = c[2] * r;
+}
+
+int alloc_waves(xnh_ctx* c) {
+  free_all(c->wave_allocs);
+  const int npix = c->cfg.width * c->cfg.height;
+  const size_t cap = static_cast<size_t>(npix) * c->nsamples;
+  WaveBuffers& B = c->B;
+  B = WaveBuffers{};
+  B.capacity = static_cast<int32_t>(cap);
+  int rc = XNH_OK;
+  for (int k = 0; k < 2 && rc == XNH_OK; ++k) {
+    rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray0, cap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray1, cap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].thr, cap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].st, cap);
+  }
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit, cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit_slot, cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh0, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh1, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh_c, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.l_term, cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.l_nee, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.accum, static_cast<size_t>(npix));
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.counters, static_cast<size_t>(kNumCtr));
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.stats, static_cast<size_t>(kNumStat));
+  if (rc == XNH_OK) {
+    const size_t spill = static_cast<size_t>(kMaxStack - c->lds_depth) * c->grid_trace * c->block;
+    rc = dev_alloc(c, c->wave_allocs, &B.spill, spill);
+  }
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &c->d_out, static_cast<size_t>(npix) * 3);
+  if (rc != XNH_OK) return rc;
+  HIP_OR_FAIL(c, hipMemset(B.accum, 0, sizeof(float4) * npix));
+  HIP_OR_FAIL(c, hipMemset(B.counters, 0, sizeof(uint32_t) * kNumCtr));
+  HIP_OR_FAIL(c, hipMemset(B.stats, 0, sizeof(unsigned long long) * kNumStat));
+  return XNH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int xnh_create(const xnh_config* cfg, xnh_ctx** out) {
+  if (!cfg || !out) return XNH_E_INVALID;
+  *out = nullptr;
+  if (cfg->width <= 0 || cfg->height <= 0 || cfg->max_bounces < 0 || cfg->min_bounces < 0) return XNH_E_INVALID;
+  if (cfg->max_bounces > 64) return XNH_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return XNH_E_NO_DEVICE;
+  xnh_ctx* c = new (std::nothrow) xnh_ctx();
+  if (!c) return XNH_E_ALLOC;
+  c->cfg = *cfg;
+  c->block = cfg->block_size > 0 ? cfg->block_size : 256;
+  if (c->block % 64 != 0 || c->block > 256) {
+    delete c;
+    return XNH_E_INVALID;
+  }
+  c->lds_depth = cfg->lds_stack > 0 ? std::min(cfg->lds_stack, kMaxStack) : 32;
+  const int npix = cfg->width * cfg->height;
+  if (cfg->spp_per_launch > 0) {
+    c->nsamples = cfg->spp_per_launch;
+  } else {
+    const int target = 8 << 20;   // paths in flight per batch
+    c->nsamples = std::max(1, std::min(64, target / npix));
+  }
+  if (static_cast<long long>(npix) * c->nsamples > (1LL << 30)) {
+    delete c;
+    return XNH_E_INVALID;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+    xnh_destroy(c);
+    return XNH_E_HIP;
+  }
+  c->grid_trace = trace_grid(c->block, c->lds_depth);
+  const int cus = num_cus();
+  if (c->grid_trace <= 0 || cus <= 0) {
+    xnh_destroy(c);
+    return XNH_E_HIP;
+  }
+  c->grid_shade = cus * 8;
+  const int rc = alloc_waves(c);
+  if (rc != XNH_OK) {
+    xnh_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return XNH_OK;
+}
+
+void xnh_destroy(xnh_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_all(c->scene_allocs);
+  free_all(c->wave_allocs);
+  for (hipEvent_t e : c->prof_events) hipEventDestroy(e);
+  if (c->ev_begin) hipEventDestroy(c->ev_begin);
+  if (c->ev_end) hipEventDestroy(c->ev_end);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* xnh_last_error(const xnh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int xnh_upload_scene(xnh_ctx* c, const xnh_flat_scene* fs) {
+  if (!c || !fs) return XNH_E_INVALID;
+  if (fs->num_meshes < 0 || fs->num_instances < 0 || fs->num_materials < 0 || fs->num_lights < 0 ||
+      fs->num_dir_lights < 0 || fs->num_tlas_nodes < 0)
+    return fail(c, XNH_E_INVALID, "negative count in flat scene");
+  const int nm = fs->num_meshes;
+  std::vector<uint32_t> node_base(nm), slot_base(nm), tri_base(nm), vert_base(nm);
+  size_t nn = 0, ns = 0, nv = 0;
+  for (int m = 0; m < nm; ++m) {
+    const xnh_mesh& me = fs->meshes[m];
+    if (me.num_triangles < 0 || me.num_vertices < 0 || me.num_nodes < 0)
+      return fail(c, XNH_E_INVALID, "negative mesh count");
+    if (me.num_triangles > 0 && me.num_nodes < 1) return fail(c, XNH_E_INVALID, "mesh without BLAS nodes");
+    node_base[m] = static_cast<uint32_t>(nn);
+    slot_base[m] = static_cast<uint32_t>(ns);
+    tri_base[m] = static_cast<uint32_t>(ns);
+    vert_base[m] = static_cast<uint32_t>(nv);
+    nn += static_cast<size_t>(me.num_nodes);
+    ns += static_cast<size_t>(me.num_triangles);
+    nv += static_cast<size_t>(me.num_vertices);
+  }
+  if (ns >= (1u << 31)) return fail(c, XNH_E_INVALID, "too many triangles");
+  std::vector<float4> bnodes(2 * nn), tris(3 * ns), vnrm(nv);
+  std::vector<int32_t> vidx(3 * ns), tmat(ns);
+  std::vector<bool> flat(nm, false);
+  for (int m = 0; m < nm; ++m) {
+    const xnh_mesh& me = fs->meshes[m];
+    for (int j = 0; j < me.num_nodes; ++j) {
+      const xnh_bvh_node& n = me.nodes[j];
+      const uint32_t child = n.count > 0 ? slot_base[m] + n.child : node_base[m] + n.child;
+      if (n.count > 0 && n.child + n.count > static_cast<uint32_t>(me.num_triangles))
+        return fail(c, XNH_E_INVALID, "BLAS leaf outside prim range");
+      // children after their parent (the builders' pre-order allocation):
+      // every walk then moves to strictly larger indices and must end
+      if (n.count == 0 && (n.child + 1 >= static_cast<uint32_t>(me.num_nodes) || n.child <= static_cast<uint32_t>(j)))
+        return fail(c, XNH_E_INVALID, "BLAS child outside node range");
+      bnodes[2 * (node_base[m] + j)] = f4(n.lo[0], n.lo[1], n.lo[2], n.hi[0]);
+      bnodes[2 * (node_base[m] + j) + 1] = f4(n.hi[1], n.hi[2], bits(child), bits(n.count));
+    }
+    for (int s = 0; s < me.num_triangles; ++s) {
+      const uint32_t t = me.prims[s];
+      if (t >= static_cast<uint32_t>(me.num_triangles)) return fail(c, XNH_E_INVALID, "prim index out of range");
+      int iv[3];
+      for (int k = 0; k < 3; ++k) {
+        iv[k] = me.indices[3 * t + k];
+        if (iv[k] < 0 || iv[k] >= me.num_vertices) return fail(c, XNH_E_INVALID, "vertex index out of range");
+      }
+      const float v0[3] = {me.vx[iv[0]], me.vy[iv[0]], me.vz[iv[0]]};
+      const float v1[3] = {me.vx[iv[1]], me.vy[iv[1]], me.vz[iv[1]]};
+      const float v2[3] = {me.vx[iv[2]], me.vy[iv[2]], me.vz[iv[2]]};
+      const float e1[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+      const float e2[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+      const size_t g = slot_base[m] + s;
+      tris[3 * g] = f4(v0[0], v0[1], v0[2], e1[0]);
+      tris[3 * g + 1] = f4(e1[1], e1[2], e2[0], e2[1]);
+      tris[3 * g + 2] = f4(e2[2], bits(t), 0.f, 0.f);
+    }
+    for (int t = 0; t < me.num_triangles; ++t) {
+      for (int k = 0; k < 3; ++k) vidx[3 * (tri_base[m] + t) + k] = me.indices[3 * t + k] + static_cast<int32_t>(vert_base[m]);
+      const int mid = me.mat_ids[t];
+      if (mid < 0 || mid >= fs->num_materials) return fail(c, XNH_E_INVALID, "material id out of range");
+      tmat[tri_base[m] + t] = mid;
+    }
+    flat[m] = !(me.nx && me.ny && me.nz);
+    for (int v = 0; v < me.num_vertices; ++v)
+      vnrm[vert_base[m] + v] = flat[m] ? f4(0.f, 0.f, 0.f, 0.f) : f4(me.nx[v], me.ny[v], me.nz[v], 0.f);
+  }
+  // TLAS nodes with their child-order key
+  const int ntn = fs->num_tlas_nodes;
+  std::vector<float4> tnodes(3 * static_cast<size_t>(ntn));
+  for (int j = 0; j < ntn; ++j) {
+    const xnh_bvh_node& n = fs->tlas_nodes[j];
+    tnodes[3 * j] = f4(n.lo[0], n.lo[1], n.lo[2], n.hi[0]);
+    tnodes[3 * j + 1] = f4(n.hi[1], n.hi[2], bits(n.child), bits(n.count));
+    float4 key = f4(0.f, 0.f, bits(0u), 0.f);
+    if (n.count == 0) {
+      if (n.child + 1 >= static_cast<uint32_t>(ntn) || n.child <= static_cast<uint32_t>(j))
+        return fail(c, XNH_E_INVALID, "TLAS child out of range");
+      const float ex = n.hi[0] - n.lo[0], ey = n.hi[1] - n.lo[1], ez = n.hi[2] - n.lo[2];
+      const int ax = (ex > ey && ex > ez) ? 0 : (ey > ez ? 1 : 2);
+      const xnh_bvh_node& l = fs->tlas_nodes[n.child];
+      const xnh_bvh_node& r = fs->tlas_nodes[n.child + 1];
+      key = f4((l.lo[ax] + l.hi[ax]) * 0.5f, (r.lo[ax] + r.hi[ax]) * 0.5f, bits(static_cast<uint32_t>(ax)), 0.f);
+    } else if (n.child + n.count > static_cast<uint32_t>(fs->num_instances)) {
+      return fail(c, XNH_E_INVALID, "TLAS leaf outside instance range");
+    }
+    tnodes[3 * j + 2] = key;
+  }
+  std::vector<uint32_t> torder(fs->tlas_inst_indices, fs->tlas_inst_indices + fs->num_instances);
+  // per-instance light lookup tables: first light naming (instance id, triangle)
+  std::vector<int32_t> lmap;
+  std::vector<float4> inst(4 * static_cast<size_t>(fs->num_instances));
+  for (int i = 0; i < fs->num_instances; ++i) {
+    const xnh_instance& in = fs->instances[i];
+    if (in.mesh < 0 || in.mesh >= nm) return fail(c, XNH_E_INVALID, "instance mesh out of range");
+    if (torder.size() > 0 && torder[i] >= static_cast<uint32_t>(fs->num_instances))
+      return fail(c, XNH_E_INVALID, "TLAS instance index out of range");
+    const float* w = in.world_to_local;
+    int32_t lm = -1;
+    const int ntri = fs->meshes[in.mesh].num_triangles;
+    for (int l = 0; l < fs->num_lights; ++l) {
+      if (static_cast<int64_t>(fs->lights[l].mesh_id) != static_cast<int64_t>(in.instance_id)) continue;
+      if (lm < 0) {
+        lm = static_cast<int32_t>(lmap.size());
+        lmap.resize(lmap.size() + std::max(ntri, 1), -1);
+      }
+      const uint32_t t = fs->lights[l].tri_idx;
+      if (t < static_cast<uint32_t>(ntri) && lmap[lm + t] < 0) lmap[lm + t] = l;
+    }
+    const uint32_t root = fs->meshes[in.mesh].num_nodes > 0 ? node_base[in.mesh] : 0xFFFFFFFFu;
+    inst[4 * i] = f4(w[0], w[1], w[2], w[3]);
+    inst[4 * i + 1] = f4(w[4], w[5], w[6], w[7]);
+    inst[4 * i + 2] = f4(w[8], w[9], w[10], w[11]);
+    inst[4 * i + 3] = f4(bits(root), ibits(in.instance_id), bits(tri_base[in.mesh] | (flat[in.mesh] ? kFlatBit : 0u)),
+                         ibits(lm));
+  }
+  // lights: corners in the coordinates of mesh `mesh_id` (cpu_tiled.cpp:93-106)
+  std::vector<float4> lights(4 * static_cast<size_t>(fs->num_lights));
+  for (int l = 0; l < fs->num_lights; ++l) {
+    const xnh_light& L = fs->lights[l];
+    if (L.mesh_id >= static_cast<uint32_t>(nm)) return fail(c, XNH_E_INVALID, "light mesh out of range");
+    const xnh_mesh& me = fs->meshes[L.mesh_id];
+    if (L.tri_idx >= static_cast<uint32_t>(me.num_triangles)) return fail(c, XNH_E_INVALID, "light triangle out of range");
+    float p[3][3];
+    for (int k = 0; k < 3; ++k) {
+      const int v = me.indices[3 * L.tri_idx + k];
+      p[k][0] = me.vx[v];
+      p[k][1] = me.vy[v];
+      p[k][2] = me.vz[v];
+    }
+    float n[3];
+    face_normal(p[0], p[1], p[2], n);
+    lights[4 * l] = f4(p[0][0], p[0][1], p[0][2], L.area);
+    lights[4 * l + 1] = f4(p[1][0], p[1][1], p[1][2], L.emission[0]);
+    lights[4 * l + 2] = f4(p[2][0], p[2][1], p[2][2], L.emission[1]);
+    lights[4 * l + 3] = f4(n[0], n[1], n[2], L.emission[2]);
+  }
+  std::vector<float4> mats(4 * static_cast<size_t>(fs->num_materials));
+  for (int i = 0; i < fs->num_materials; ++i) {
+    const xnh_material& m = fs->materials[i];
+    mats[4 * i] = f4(m.base_color[0], m.base_color[1], m.base_color[2], m.metallic);
+    mats[4 * i + 1] = f4(m.ior, m.alpha, m.alpha_x, m.alpha_y);
+    mats[4 * i + 2] = f4(m.f0[0], m.f0[1], m.f0[2], m.subsurface);
+    mats[4 * i + 3] = f4(m.subsurface_color[0], m.subsurface_color[1], m.subsurface_color[2], bits(m.flags));
+  }
+  std::vector<float4> suns(2 * static_cast<size_t>(fs->num_dir_lights));
+  for (int i = 0; i < fs->num_dir_lights; ++i) {
+    const xnh_dir_light& d = fs->dir_lights[i];
+    suns[2 * i] = f4(d.direction[0], d.direction[1], d.direction[2], 0.f);
+    suns[2 * i + 1] = f4(d.emission[0], d.emission[1], d.emission[2], 0.f);
+  }
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  free_all(c->scene_allocs);
+  c->has_scene = false;
+  DevScene S{};
+  float4 *p_bn, *p_tr, *p_tn, *p_in, *p_vn, *p_li, *p_ma, *p_su;
+  uint32_t* p_to;
+  int32_t *p_vi, *p_tm, *p_lm;
+  int rc = dev_upload(c, c->scene_allocs, &p_bn, bnodes);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_tr, tris);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_tn, tnodes);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_to, torder);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_in, inst);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_vi, vidx);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_vn, vnrm);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_tm, tmat);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_lm, lmap);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_li, lights);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_ma, mats);
+  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_su, suns);
+  if (rc != XNH_OK) {
+    free_all(c->scene_allocs);
+    return rc;
+  }
+  S.bnodes = p_bn;
+  S.tris = p_tr;
+  S.tnodes = p_tn;
+  S.torder = p_to;
+  S.inst = p_in;
+  S.vidx = p_vi;
+  S.vnrm = p_vn;
+  S.tmat = p_tm;
+  S.lmap = p_lm;
+  S.lights = p_li;
+  S.mats = p_ma;
+  S.suns = p_su;
+  S.num_tnodes = ntn;
+  S.num_lights = fs->num_lights;
+  S.num_suns = fs->num_dir_lights;
+  for (int k = 0; k < 3; ++k) {
+    S.sky.zenith[k] = fs->sky.zenith[k];
+    S.sky.horizon[k] = fs->sky.horizon[k];
+    S.sky.ground[k] = fs->sky.ground[k];
+  }
+  S.sky.sharpness = fs->sky.sharpness;
+  S.sky.intensity = fs->sky.intensity;
+  c->S = S;
+  c->has_scene = true;
+  return XNH_OK;
+}
+
+int xnh_set_profiling(xnh_ctx* c, int32_t on) {
+  if (!c) return XNH_E_INVALID;
+  c->profiling = on != 0;
+  return XNH_OK;
+}
+
+int xnh_render(xnh_ctx* c, const xnh_camera* cam, int32_t spp_first, int32_t spp_count, float* rgb_out) {
+  if (!c || !cam || spp_count < 0 || spp_first < 0) return XNH_E_INVALID;
+  if (!c->has_scene) return fail(c, XNH_E_NO_SCENE, "xnh_render before xnh_upload_scene");
+  const int W = c->cfg.width, H = c->cfg.height, npix = W * H;
+  const int maxb = c->cfg.max_bounces, minb = c->cfg.min_bounces;
+  DevCamera dc;
+  for (int k = 0; k < 3; ++k) {
+    dc.eye[k] = cam->eye[k];
+    dc.llc[k] = cam->lower_left[k];
+    dc.horiz[k] = cam->horizontal[k];
+    dc.vert[k] = cam->vertical[k];
+  }
+  hipStream_t s = c->stream;
+  HIP_OR_FAIL(c, hipMemsetAsync(c->B.stats, 0, sizeof(unsigned long long) * kNumStat, s));
+  HIP_OR_FAIL(c, hipMemsetAsync(c->B.counters + kOverflow, 0, sizeof(uint32_t), s));
+  // per-kernel timing: an event before and after each traversal/shade launch
+  const int nbatches = spp_count == 0 ? 0 : (spp_count + c->nsamples - 1) / c->nsamples;
+  const size_t nev = c->profiling ? static_cast<size_t>(nbatches) * maxb * 6 : 0;
+  while (c->prof_events.size() < nev) {
+    hipEvent_t e;
+    HIP_OR_FAIL(c, hipEventCreate(&e));
+    c->prof_events.push_back(e);
+  }
+  HIP_OR_FAIL(c, hipEventRecord(c->ev_begin, s));
+  size_t ev = 0;
+  int launches = 0;
+  for (int done = 0; done < spp_count;) {
+    const int ns = std::min(c->nsamples, spp_count - done);
+    HIP_OR_FAIL(c, launch_raygen(s, c->B, dc, W, H, spp_first + done, ns, 2 * std::max(maxb, 1)));
+    for (int d = 0; d < maxb; ++d) {
+      const int qi = d & 1;
+      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
+      HIP_OR_FAIL(c, launch_closest(s, c->S, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth));
+      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
+      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
+      HIP_OR_FAIL(c, launch_shade(s, c->S, c->B, qi, d, minb, maxb, c->grid_shade));
+      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
+      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
+      HIP_OR_FAIL(c, launch_shadow(s, c->S, c->B, 2 * d + 1, c->grid_trace, c->block, c->lds_depth));
+      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
+      ++launches;
+    }
+    HIP_OR_FAIL(c, launch_accumulate(s, c->B, npix, ns));
+    done += ns;
+  }
+  HIP_OR_FAIL(c, hipEventRecord(c->ev_end, s));
+  c->spp_total += spp_count;
+  if (rgb_out && c->spp_total > 0) {
+    const float inv = 1.0f / static_cast<float>(c->spp_total);
+    HIP_OR_FAIL(c, launch_average(s, c->B.accum, c->d_out, npix, inv));
+    HIP_OR_FAIL(c, hipMemcpyAsync(rgb_out, c->d_out, sizeof(float) * 3 * npix, hipMemcpyDeviceToHost, s));
+  }
+  unsigned long long st[kNumStat];
+  uint32_t ovf = 0;
+  HIP_OR_FAIL(c, hipMemcpyAsync(st, c->B.stats, sizeof st, hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(c, hipMemcpyAsync(&ovf, c->B.counters + kOverflow, sizeof ovf, hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(c, hipStreamSynchronize(s));
+  xnh_stats& o = c->stats;
+  o = xnh_stats{};
+  o.paths = static_cast<uint64_t>(npix) * spp_count;
+  o.closest_rays = st[kStClosest];
+  o.shadow_rays = st[kStShadow];
+  o.node_tests = st[kStNodeTests];
+  o.interior = st[kStInterior];
+  o.tri_tests = st[kStTriTests];
+  o.inst_visits = st[kStInstVisits];
+  o.nonfinite = st[kStNonFinite];
+  float ms = 0.f;
+  HIP_OR_FAIL(c, hipEventElapsedTime(&ms, c->ev_begin, c->ev_end));
+  o.render_ms = ms;
+  o.closest_launches = launches;
+  o.stack_overflows = static_cast<int32_t>(ovf);
+  if (c->profiling) {
+    for (size_t k = 0; k + 5 < ev; k += 6) {
+      float a = 0.f, b = 0.f, d = 0.f;
+      HIP_OR_FAIL(c, hipEventElapsedTime(&a, c->prof_events[k], c->prof_events[k + 1]));
+      HIP_OR_FAIL(c, hipEventElapsedTime(&b, c->prof_events[k + 2], c->prof_events[k + 3]));
+      HIP_OR_FAIL(c, hipEventElapsedTime(&d, c->prof_events[k + 4], c->prof_events[k + 5]));
+      o.closest_ms += a;
+      o.shade_ms += b;
+      o.shadow_ms += d;
+    }
+  }
+  if (ovf) return fail(c, XNH_E_OVERFLOW, "traversal stack overflow");
+  return XNH_OK;
+}
+
+int xnh_read_accum(xnh_ctx* c, float* sum_out, int32_t* spp_out) {
+  if (!c) return XNH_E_INVALID;
+  const int npix = c->cfg.width * c->cfg.height;
+  if (sum_out) {
+    std::vector<float4> tmp(npix);
+    HIP_OR_FAIL(c, hipMemcpyAsync(tmp.data(), c->B.accum, sizeof(float4) * npix, hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+    for (int p = 0; p < npix; ++p) {
+      sum_out[3 * p] = tmp[p].x;
+      sum_out[3 * p + 1] = tmp[p].y;
+      sum_out[3 * p + 2] = tmp[p].z;
+    }
+  }
+  if (spp_out) *spp_out = c->spp_total;
+  return XNH_OK;
+}
+
+int xnh_reset(xnh_ctx* c) {
+  if (!c) return XNH_E_INVALID;
+  HIP_OR_FAIL(c, hipMemsetAsync(c->B.accum, 0, sizeof(float4) * c->cfg.width * c->cfg.height, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  c->spp_total = 0;
+  return XNH_OK;
+}
+
+int xnh_spp(const xnh_ctx* c) { return c ? c->spp_total : XNH_E_INVALID; }
+
+int xnh_get_stats(const xnh_ctx* c, xnh_stats* out) {
+  if (!c || !out) return XNH_E_INVALID;
+  *out = c->stats;
+  return XNH_OK;
+}
+
+static int trace_common(xnh_ctx* c, const xnh_ray* rays, int32_t n, bool any, void* out) {
+  if (!c || (n > 0 && (!rays || !out)) || n < 0) return XNH_E_INVALID;
+  if (!c->has_scene) return fail(c, XNH_E_NO_SCENE, "trace before xnh_upload_scene");
+  c->stats = xnh_stats{};
+  if (n == 0) return XNH_OK;
+  std::vector<float4> hr(2 * static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    hr[2 * i] = f4(rays[i].org[0], rays[i].org[1], rays[i].org[2], rays[i].tmin);
+    hr[2 * i + 1] = f4(rays[i].dir[0], rays[i].dir[1], rays[i].dir[2], rays[i].tmax);
+  }
+  std::vector<void*> tmp;
+  float4* d_rays = nullptr;
+  void* d_out = nullptr;
+  uint32_t* d_spill = nullptr;
+  unsigned long long* d_stats = nullptr;
+  uint32_t* d_ovf = nullptr;
+  const int block = 256;
+  const size_t threads = static_cast<size_t>((n + block - 1) / block) * block;
+  int rc = dev_upload(c, tmp, &d_rays, hr);
+  if (rc == XNH_OK) {
+    if (any) rc = dev_alloc(c, tmp, reinterpret_cast<uint8_t**>(&d_out), static_cast<size_t>(n));
+    else rc = dev_alloc(c, tmp, reinterpret_cast<float**>(&d_out), static_cast<size_t>(n) * 16);
+  }
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &d_spill, static_cast<size_t>(kMaxStack - c->lds_depth) * threads);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &d_stats, static_cast<size_t>(kNumStat));
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &d_ovf, 1);
+  if (rc != XNH_OK) {
+    free_all(tmp);
+    return rc;
+  }
+  hipError_t e = hipMemsetAsync(d_stats, 0, sizeof(unsigned long long) * kNumStat, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_ovf, 0, sizeof(uint32_t), c->stream);
+  if (e == hipSuccess) e = hipEventRecord(c->ev_begin, c->stream);
+  if (e == hipSuccess) {
+    e = any ? launch_api_any(c->stream, c->S, d_rays, static_cast<uint8_t*>(d_out), n, d_spill, c->lds_depth, d_stats,
+                             d_ovf)
+            : launch_api_closest(c->stream, c->S, d_rays, static_cast<float*>(d_out), n, d_spill, c->lds_depth,
+                                 d_stats, d_ovf);
+  }
+  if (e == hipSuccess) e = hipEventRecord(c->ev_end, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out, d_out, any ? static_cast<size_t>(n) : static_cast<size_t>(n) * 64, hipMemcpyDeviceToHost,
+                       c->stream);
+  unsigned long long st[kNumStat] = {};
+  uint32_t ovf = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(st, d_stats, sizeof st, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(&ovf, d_ovf, sizeof ovf, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev_begin, c->ev_end);
+  free_all(tmp);
+  if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("trace: ") + hipGetErrorString(e));
+  c->stats.closest_rays = any ? 0 : st[kStClosest];
+  c->stats.shadow_rays = any ? st[kStClosest] : 0;
+  c->stats.node_tests = st[kStNodeTests];
+  c->stats.interior = st[kStInterior];
+  c->stats.tri_tests = st[kStTriTests];
+  c->stats.inst_visits = st[kStInstVisits];
+  c->stats.render_ms = ms;
+  c->stats.stack_overflows = static_cast<int32_t>(ovf);
+  if (ovf) return fail(c, XNH_E_OVERFLOW, "traversal stack overflow");
+  return XNH_OK;
+}
+
+int xnh_trace_closest(xnh_ctx* c, const xnh_ray* rays, int32_t n, xnh_hit* hits) {
+  return trace_common(c, rays, n, false, hits);
+}
+int xnh_trace_any(xnh_ctx* c, const xnh_ray* rays, int32_t n, uint8_t* occ) {
+  return trace_common(c, rays, n, true, occ);
+}
+
+// ---- host scene builder ----------------------------------------------------
+
+xnh_scene* xnh_scene_new(void) { return new (std::nothrow) xnh_scene(); }
+void xnh_scene_free(xnh_scene* s) { delete s; }
+
+int xnh_scene_add_material(xnh_scene* s, const float* p) {
+  if (!s || !p) return XNH_E_INVALID;
+  s->s.materials.push_back(prepare_material(p));
+  s->s.built = false;
+  return static_cast<int>(s->s.materials.size()) - 1;
+}
+
+int xnh_scene_add_mesh(xnh_scene* s, int32_t nv, const float* pos, const float* nrm, int32_t nt,
+                       const int32_t* indices, const int32_t* mat_ids) {
+  if (!s || nv < 0 || nt < 0 || (nv > 0 && !pos) || (nt > 0 && (!indices || !mat_ids))) return XNH_E_INVALID;
+  HostMesh m;
+  m.vx.resize(nv);
+  m.vy.resize(nv);
+  m.vz.resize(nv);
+  for (int i = 0; i < nv; ++i) {
+    m.vx[i] = pos[3 * i];
+    m.vy[i] = pos[3 * i + 1];
+    m.vz[i] = pos[3 * i + 2];
+  }
+  m.indices.assign(indices, indices + 3 * static_cast<size_t>(nt));
+  m.mat_ids.assign(mat_ids, mat_ids + nt);
+  for (int32_t v : m.indices)
+    if (v < 0 || v >= nv) return XNH_E_INVALID;
+  if (nrm) {
+    m.nx.resize(nv);
+    m.ny.resize(nv);
+    m.nz.resize(nv);
+    for (int i = 0; i < nv; ++i) {
+      m.nx[i] = nrm[3 * i];
+      m.ny[i] = nrm[3 * i + 1];
+      m.nz[i] = nrm[3 * i + 2];
+    }
+  } else {
+    smooth_normals(m);
+  }
+  s->s.meshes.push_back(std::move(m));
+  s->s.built = false;
+  return static_cast<int>(s->s.meshes.size()) - 1;
+}
+
+int xnh_scene_add_light(xnh_scene* s, int32_t mesh, int32_t tri, const float* e, float area) {
+  if (!s || !e || mesh < 0 || tri < 0) return XNH_E_INVALID;
+  xnh_light l{};
+  l.mesh_id = static_cast<uint32_t>(mesh);
+  l.tri_idx = static_cast<uint32_t>(tri);
+  l.emission[0] = e[0];
+  l.emission[1] = e[1];
+  l.emission[2] = e[2];
+  l.area = area;
+  s->s.lights.push_back(l);
+  return static_cast<int>(s->s.lights.size()) - 1;
+}
+
+// Normalised as the scene parser does (scene_file.cpp:137-140).
+int xnh_scene_add_dir_light(xnh_scene* s, const float* d, const float* color, float intensity) {
+  if (!s || !d || !color) return XNH_E_INVALID;
+  xnh_dir_light l{};
+  const float len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  if (len > 1e-6f) {
+    const float r = 1.f / len;
+    l.direction[0] = d[0] * r;
+    l.direction[1] = d[1] * r;
+    l.direction[2] = d[2] * r;
+  } else {
+    l.direction[0] = 0.f;
+    l.direction[1] = 1.f;
+    l.direction[2] = 0.f;
+  }
+  for (int k = 0; k < 3; ++k) l.emission[k] = color[k] * intensity;
+  s->s.suns.push_back(l);
+  return static_cast<int>(s->s.suns.size()) - 1;
+}
+
+int xnh_scene_set_sky(xnh_scene* s, const float* z, const float* h, const float* g, float intensity, float sharp) {
+  if (!s || !z || !h || !g) return XNH_E_INVALID;
+  for (int k = 0; k < 3; ++k) {
+    s->s.sky.zenith[k] = z[k];
+    s->s.sky.horizon[k] = h[k];
+    s->s.sky.ground[k] = g[k];
+  }
+  s->s.sky.intensity = intensity;
+  s->s.sky.sharpness = sharp;
+  return XNH_OK;
+}
+
+int xnh_scene_add_instance(xnh_scene* s, int32_t mesh, const float* t, float scale, const float* rot, int32_t id) {
+  if (!s || !t || !rot || mesh < 0) return XNH_E_INVALID;
+  PendingInstance p{};
+  p.mesh = mesh;
+  p.id = id;
+  p.s = scale;
+  for (int k = 0; k < 3; ++k) {
+    p.t[k] = t[k];
+    p.rot[k] = rot[k];
+  }
+  s->s.pending.push_back(p);
+  s->s.built = false;
+  return static_cast<int>(s->s.pending.size()) - 1;
+}
+
+int xnh_scene_build(xnh_scene* s, int32_t nthreads) {
+  (void)nthreads;
+  if (!s) return XNH_E_INVALID;
+  HostScene& hs = s->s;
+  for (const auto& p : hs.pending)
+    if (p.mesh >= static_cast<int32_t>(hs.meshes.size())) return XNH_E_INVALID;
+  for (auto& m : hs.meshes) build_blas(m);
+  build_tlas(hs);
+  hs.mesh_views.resize(hs.meshes.size());
+  for (size_t i = 0; i < hs.meshes.size(); ++i) {
+    HostMesh& m = hs.meshes[i];
+    xnh_mesh& v = hs.mesh_views[i];
+    v.num_vertices = static_cast<int32_t>(m.vx.size());
+    v.num_triangles = m.tris();
+    v.vx = m.vx.data();
+    v.vy = m.vy.data();
+    v.vz = m.vz.data();
+    v.nx = m.nx.empty() ? nullptr : m.nx.data();
+    v.ny = m.ny.empty() ? nullptr : m.ny.data();
+    v.nz = m.nz.empty() ? nullptr : m.nz.data();
+    v.indices = m.indices.data();
+    v.mat_ids = m.mat_ids.data();
+    v.num_nodes = static_cast<int32_t>(m.nodes.size());
+    v.nodes = m.nodes.data();
+    v.prims = m.prims.data();
+  }
+  hs.built = true;
+  return XNH_OK;
+}
+
+int xnh_scene_flat(const xnh_scene* s, xnh_flat_scene* out) {
+  if (!s || !out || !s->s.built) return XNH_E_INVALID;
+  const HostScene& hs = s->s;
+  out->num_meshes = static_cast<int32_t>(hs.mesh_views.size());
+  out->meshes = hs.mesh_views.data();
+  out->num_instances = static_cast<int32_t>(hs.instances.size());
+  out->instances = hs.instances.data();
+  out->num_tlas_nodes = static_cast<int32_t>(hs.tlas_nodes.size());
+  out->tlas_nodes = hs.tlas_nodes.data();
+  out->tlas_inst_indices = hs.tlas_order.data();
+  out->num_materials = static_cast<int32_t>(hs.materials.size());
+  out->materials = hs.materials.data();
+  out->num_lights = static_cast<int32_t>(hs.lights.size());
+  out->lights = hs.lights.data();
+  out->num_dir_lights = static_cast<int32_t>(hs.suns.size());
+  out->dir_lights = hs.suns.data();
+  out->sky = hs.sky;
+  return XNH_OK;
+}
+
+int xnh_camera_look_at(const float* eye, const float* target, const float* up, float fov, float aspect,
+                       xnh_camera* out) {
+  if (!eye || !target || !up || !out) return XNH_E_INVALID;
+  *out = look_at(eye, target, up, fov, aspect);
+  return XNH_OK;
+}
+
+}  // extern "C"

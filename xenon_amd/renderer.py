@@ -1,0 +1,209 @@
+"""Python host mirror of the reference's GPU-backend contract.
+
+``HipRenderer`` mirrors ``CudaRenderer`` (src/cuda/cuda_renderer.cuh:14-55):
+constructed with (width, height, max_bounces, min_bounces), it takes a built
+scene once through ``upload_scene`` and adds exactly one sample per
+``render_frame``; ``spp()`` reports the count. ``HipScene`` plays the role of
+the reference's ``Scene`` after ``build_acceleration`` (scene/scene.h:79-121).
+Every call goes through libxenon_hip.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import native as N
+from .scenes import SceneDesc
+
+NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("child", "<u4"), ("count", "<u4")])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("instance_id", "<i4"), ("prim_id", "<i4"),
+                      ("mat_id", "<i4"), ("pos", "<f4", 3), ("normal", "<f4", 3), ("geo_normal", "<f4", 3),
+                      ("front_face", "<i4")])
+assert HIT_DTYPE.itemsize == C.sizeof(N.Hit)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class HipScene:
+    """Scene built by the backend's host builder from a ``SceneDesc``."""
+
+    def __init__(self, desc: SceneDesc):
+        L = N.lib()
+        self.desc = desc
+        self.h = L.xnh_scene_new()
+        if not self.h:
+            raise MemoryError("xnh_scene_new")
+        for m in desc.materials:
+            N.check(L.xnh_scene_add_material(self.h, _fp(m.params16())))
+        for m in desc.meshes:
+            pos = np.ascontiguousarray(m.positions, np.float32)
+            idx = np.ascontiguousarray(m.indices, np.int32)
+            mat = np.ascontiguousarray(m.mat_ids, np.int32)
+            nrm = None if m.normals is None else np.ascontiguousarray(m.normals, np.float32)
+            N.check(L.xnh_scene_add_mesh(self.h, pos.shape[0], _fp(pos), _fp(nrm) if nrm is not None else None,
+                                         idx.shape[0], _ip(idx), _ip(mat)))
+        for lt in desc.lights:
+            e = np.array(lt.emission, np.float32)
+            N.check(L.xnh_scene_add_light(self.h, lt.mesh, lt.tri, _fp(e), float(np.float32(lt.area))))
+        for s in desc.suns:
+            d = np.array(s.direction, np.float32)
+            c = np.array(s.color, np.float32)
+            N.check(L.xnh_scene_add_dir_light(self.h, _fp(d), _fp(c), s.intensity))
+        sk = desc.sky
+        z, hz, g = (np.array(v, np.float32) for v in (sk.zenith, sk.horizon, sk.ground))
+        N.check(L.xnh_scene_set_sky(self.h, _fp(z), _fp(hz), _fp(g), sk.intensity, sk.sharpness))
+        for ins in desc.instances:
+            t = np.array(ins.translation, np.float32)
+            r = np.array(ins.rot_deg, np.float32)
+            N.check(L.xnh_scene_add_instance(self.h, ins.mesh, _fp(t), ins.scale, _fp(r), ins.instance_id))
+        N.check(L.xnh_scene_build(self.h, 0))
+        self._flat = N.FlatScene()
+        N.check(L.xnh_scene_flat(self.h, C.byref(self._flat)))
+
+    def __del__(self):
+        try:
+            if self.h:
+                N.lib().xnh_scene_free(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def flat(self) -> N.FlatScene:
+        return self._flat
+
+    def camera(self, width=None, height=None) -> N.Camera:
+        cam = self.desc.camera
+        W = width or self.desc.width
+        H = height or self.desc.height
+        aspect = float(np.float32(W) / np.float32(H))
+        e, t, u = (np.array(v, np.float32) for v in (cam.eye, cam.target, cam.up))
+        out = N.Camera()
+        N.check(N.lib().xnh_camera_look_at(_fp(e), _fp(t), _fp(u), cam.fov, aspect, C.byref(out)))
+        return out
+
+    def blas(self, mesh: int):
+        mv = self._flat.meshes[mesh]
+        nodes = np.ctypeslib.as_array(C.cast(mv.nodes, C.POINTER(C.c_uint8)), shape=(mv.num_nodes * 32,))
+        nodes = nodes.view(NODE_DTYPE).copy()
+        prims = np.ctypeslib.as_array(mv.prims, shape=(mv.num_triangles,)).copy()
+        return nodes, prims
+
+    def normals(self, mesh: int) -> np.ndarray:
+        mv = self._flat.meshes[mesh]
+        n = mv.num_vertices
+        return np.stack([np.ctypeslib.as_array(p, shape=(n,)) for p in (mv.nx, mv.ny, mv.nz)], 1).copy()
+
+    def tlas(self):
+        f = self._flat
+        nodes = np.ctypeslib.as_array(C.cast(f.tlas_nodes, C.POINTER(C.c_uint8)), shape=(f.num_tlas_nodes * 32,))
+        nodes = nodes.view(NODE_DTYPE).copy()
+        order = np.ctypeslib.as_array(f.tlas_inst_indices, shape=(f.num_instances,)).copy()
+        ni = f.num_instances
+        xf = np.zeros((ni, 24), np.float32)
+        ids = np.zeros(ni, np.int32)
+        meshes = np.zeros(ni, np.int32)
+        for i in range(ni):
+            r = f.instances[i]
+            xf[i, :12] = np.array(r.local_to_world[:], np.float32)
+            xf[i, 12:] = np.array(r.world_to_local[:], np.float32)
+            ids[i] = r.instance_id
+            meshes[i] = r.mesh
+        return nodes, order, xf, ids, meshes
+
+
+class HipRenderer:
+    """The backend renderer: ``CudaRenderer``'s three-method contract plus
+    batch rendering and the ray-batch parity API."""
+
+    def __init__(self, width: int, height: int, max_bounces: int = 8, min_bounces: int = 3,
+                 spp_per_launch: int = 0, block_size: int = 0, lds_stack: int = 0):
+        self.width, self.height = int(width), int(height)
+        cfg = N.Config(self.width, self.height, int(max_bounces), int(min_bounces), int(spp_per_launch),
+                       int(block_size), int(lds_stack), 0)
+        h = C.c_void_p()
+        N.check(N.lib().xnh_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.scene = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().xnh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _chk(self, rc):
+        return N.check(rc, self.h)
+
+    # -- CudaRenderer contract ---------------------------------------------
+    def upload_scene(self, scene: HipScene) -> None:
+        self._chk(N.lib().xnh_upload_scene(self.h, C.byref(scene.flat)))
+        self.scene = scene
+
+    def render_frame(self, camera: N.Camera) -> np.ndarray:
+        """Adds exactly one sample (index spp()+1) and returns the running
+        average as the reference's swapchain holds it: (H, W, 3), row 0 at the
+        top (rows flipped, render/cpu_tiled.cpp:228-229)."""
+        img = self.render(camera, self.spp() + 1, 1)
+        return img.reshape(self.height, self.width, 3)[::-1].copy()
+
+    def spp(self) -> int:
+        return N.lib().xnh_spp(self.h)
+
+    # -- batch API ----------------------------------------------------------
+    def render(self, camera: N.Camera, spp_first: int, spp_count: int, want_image: bool = True):
+        """Adds samples spp_first..spp_first+spp_count-1 (1-based); returns the
+        running average in pixel order, (H*W, 3), or None."""
+        out = np.empty((self.height * self.width, 3), np.float32) if want_image else None
+        self._chk(N.lib().xnh_render(self.h, C.byref(camera), int(spp_first), int(spp_count),
+                                     _fp(out) if out is not None else None))
+        return out
+
+    def read_accum(self):
+        out = np.empty((self.height * self.width, 3), np.float32)
+        spp = C.c_int32()
+        self._chk(N.lib().xnh_read_accum(self.h, _fp(out), C.byref(spp)))
+        return out, spp.value
+
+    def reset(self):
+        self._chk(N.lib().xnh_reset(self.h))
+
+    def set_profiling(self, on: bool):
+        self._chk(N.lib().xnh_set_profiling(self.h, 1 if on else 0))
+
+    def stats(self) -> dict:
+        s = N.Stats()
+        self._chk(N.lib().xnh_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    # -- ray batches ----------------------------------------------------------
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros(n, HIT_DTYPE)
+        self._chk(N.lib().xnh_trace_closest(self.h, rays.ctypes.data_as(C.POINTER(N.Ray)), n,
+                                            hits.ctypes.data_as(C.POINTER(N.Hit))))
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        occ = np.zeros(n, np.uint8)
+        self._chk(N.lib().xnh_trace_any(self.h, rays.ctypes.data_as(C.POINTER(N.Ray)), n,
+                                        occ.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return occ
