@@ -161,10 +161,14 @@ typedef struct {
   uint64_t paths;            /* camera paths started */
   uint64_t closest_rays;     /* closest-hit rays traced */
   uint64_t shadow_rays;      /* shadow rays traced */
-  uint64_t node_tests;       /* BVH boxes tested (BLAS pair = 2, TLAS pop = 1) */
-  uint64_t interior;         /* interior nodes expanded */
-  uint64_t tri_tests;        /* ray-triangle tests */
-  uint64_t inst_visits;      /* instance ray transforms */
+  uint64_t node_tests;       /* closest-hit BVH boxes tested (BLAS pair = 2, TLAS pop = 1) */
+  uint64_t interior;         /* closest-hit interior nodes expanded */
+  uint64_t tri_tests;        /* closest-hit ray-triangle tests */
+  uint64_t inst_visits;      /* closest-hit instance ray transforms */
+  uint64_t shadow_node_tests;   /* the same four counts for shadow (any-hit) rays */
+  uint64_t shadow_interior;
+  uint64_t shadow_tri_tests;
+  uint64_t shadow_inst_visits;
   uint64_t nonfinite;        /* non-finite path radiances (kept, as the reference does) */
   double render_ms;          /* device time of the call (HIP events) */
   double closest_ms;         /* device time inside the closest-hit kernels */

@@ -88,13 +88,24 @@ def bounce_rays(hf, hi, n, rng):
 
 
 def shadow_rays(rays, hf, hi, rng):
-    """Segments between two points of a ray batch with finite tmax."""
+    """Segments between two surface points of a ray batch (each hit point,
+    nudged off its surface, toward the hit point of another ray) with finite
+    tmax, the shape of the integrator's area-light shadow rays; misses fall
+    back to a point two units along their ray."""
     n = rays.shape[0]
-    o = rays[:, :3]
-    target = o + rays[:, 3:6] * F32(2.0)
     has = hi[:, 0] >= 0
-    target[has] = hf[has, 3:6]
-    d = target - o
+    pts = rays[:, :3] + rays[:, 3:6] * F32(2.0)
+    pts[has] = hf[has, 3:6]
+    partner = rng.permutation(n)
+    o = pts.copy()
+    gn = hf[:, 9:12]
+    tgt = pts[partner]
+    d = tgt - o
+    flip = (gn * d).sum(1) < 0
+    g = gn.copy()
+    g[flip] = -g[flip]
+    o[has] = (o[has] + g[has] * F32(5e-4)).astype(F32)
+    d = tgt - o
     dist = np.sqrt((d * d).sum(1)).astype(F32)
     dist[dist == 0] = 1
     d = (d / dist[:, None]).astype(F32)

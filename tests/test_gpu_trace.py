@@ -78,7 +78,8 @@ def test_any_bit_exact(pair):
     print(f"  {desc.name}: {len(srays)} shadow rays, occluded {int(occ_ref.sum())}, mismatches {bad}", flush=True)
     assert bad / len(srays) <= GRAZING_MAX
     if bad == 0:
-        assert [st["node_tests"], st["interior"], st["tri_tests"], st["inst_visits"]] == [int(x) for x in cnt]
+        assert [st["shadow_node_tests"], st["shadow_interior"], st["shadow_tri_tests"],
+                st["shadow_inst_visits"]] == [int(x) for x in cnt]
 
 
 def test_empty_batch_and_errors(pair):

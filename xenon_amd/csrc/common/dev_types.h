@@ -106,6 +106,10 @@ enum Stat : int {
   kStInterior,
   kStTriTests,
   kStInstVisits,
+  kStShNodeTests,     // the same four for any-hit (shadow) traversal
+  kStShInterior,
+  kStShTriTests,
+  kStShInstVisits,
   kStNonFinite,
   kNumStat
 };

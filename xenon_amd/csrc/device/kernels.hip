@@ -51,16 +51,18 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
-__device__ __forceinline__ void flush_counts(unsigned long long* stats, const TravCount& c, uint32_t rays) {
+// Wave-reduced traversal counters; `any` selects the shadow slots.
+__device__ __forceinline__ void flush_counts(unsigned long long* stats, const TravCount& c, uint32_t rays, bool any) {
   const unsigned long long a = wave_sum(c.node_tests), b = wave_sum(c.interior), t = wave_sum(c.tri_tests),
                            i = wave_sum(c.inst_visits);
   const unsigned long long r = wave_sum(rays);
+  const int base = any ? kStShNodeTests : kStNodeTests;
   if (lane_id() == 0) {
-    if (a) atomicAdd(&stats[kStNodeTests], a);
-    if (b) atomicAdd(&stats[kStInterior], b);
-    if (t) atomicAdd(&stats[kStTriTests], t);
-    if (i) atomicAdd(&stats[kStInstVisits], i);
-    if (r) atomicAdd(&stats[kStClosest], r);
+    if (a) atomicAdd(&stats[base], a);
+    if (b) atomicAdd(&stats[base + 1], b);
+    if (t) atomicAdd(&stats[base + 2], t);
+    if (i) atomicAdd(&stats[base + 3], i);
+    if (r) atomicAdd(&stats[any ? kStShadow : kStClosest], r);
   }
 }
 
@@ -204,7 +206,7 @@ __global__ void __launch_bounds__(256) k_closest(DevScene S, WaveBuffers B, int 
       ++rays;
     }
   }
-  flush_counts(B.stats, c, rays);
+  flush_counts(B.stats, c, rays, false);
   if (overflow) atomicAdd(&B.counters[kOverflow], 1u);
 }
 
@@ -432,15 +434,7 @@ __global__ void __launch_bounds__(256) k_shadow(DevScene S, WaveBuffers B, int c
       ++rays;
     }
   }
-  const unsigned long long a = wave_sum(c.node_tests), b = wave_sum(c.interior), t = wave_sum(c.tri_tests),
-                           ii = wave_sum(c.inst_visits), r = wave_sum(rays);
-  if (lane_id() == 0) {
-    if (a) atomicAdd(&B.stats[kStNodeTests], a);
-    if (b) atomicAdd(&B.stats[kStInterior], b);
-    if (t) atomicAdd(&B.stats[kStTriTests], t);
-    if (ii) atomicAdd(&B.stats[kStInstVisits], ii);
-    if (r) atomicAdd(&B.stats[kStShadow], r);
-  }
+  flush_counts(B.stats, c, rays, true);
   if (overflow) atomicAdd(&B.counters[kOverflow], 1u);
 }
 
@@ -521,7 +515,7 @@ __global__ void __launch_bounds__(256) k_api_closest(DevScene S, const float4* r
     rays_done = 1;
     if (st.overflow) atomicAdd(overflow_ctr, 1u);
   }
-  flush_counts(stats, c, rays_done);
+  flush_counts(stats, c, rays_done, false);
 }
 
 __global__ void __launch_bounds__(256) k_api_any(DevScene S, const float4* rays, uint8_t* occ, int n, uint32_t* spill,
@@ -538,7 +532,7 @@ __global__ void __launch_bounds__(256) k_api_any(DevScene S, const float4* rays,
     rays_done = 1;
     if (st.overflow) atomicAdd(overflow_ctr, 1u);
   }
-  flush_counts(stats, c, rays_done);
+  flush_counts(stats, c, rays_done, true);
 }
 
 // ---------------------------------------------------------------------------

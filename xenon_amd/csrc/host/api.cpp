@@ -498,6 +498,10 @@ int xnh_render(xnh_ctx* c, const xnh_camera* cam, int32_t spp_first, int32_t spp
   o.interior = st[kStInterior];
   o.tri_tests = st[kStTriTests];
   o.inst_visits = st[kStInstVisits];
+  o.shadow_node_tests = st[kStShNodeTests];
+  o.shadow_interior = st[kStShInterior];
+  o.shadow_tri_tests = st[kStShTriTests];
+  o.shadow_inst_visits = st[kStShInstVisits];
   o.nonfinite = st[kStNonFinite];
   float ms = 0.f;
   HIP_OR_FAIL(c, hipEventElapsedTime(&ms, c->ev_begin, c->ev_end));
@@ -604,12 +608,16 @@ static int trace_common(xnh_ctx* c, const xnh_ray* rays, int32_t n, bool any, vo
   if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev_begin, c->ev_end);
   free_all(tmp);
   if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("trace: ") + hipGetErrorString(e));
-  c->stats.closest_rays = any ? 0 : st[kStClosest];
-  c->stats.shadow_rays = any ? st[kStClosest] : 0;
+  c->stats.closest_rays = st[kStClosest];
+  c->stats.shadow_rays = st[kStShadow];
   c->stats.node_tests = st[kStNodeTests];
   c->stats.interior = st[kStInterior];
   c->stats.tri_tests = st[kStTriTests];
   c->stats.inst_visits = st[kStInstVisits];
+  c->stats.shadow_node_tests = st[kStShNodeTests];
+  c->stats.shadow_interior = st[kStShInterior];
+  c->stats.shadow_tri_tests = st[kStShTriTests];
+  c->stats.shadow_inst_visits = st[kStShInstVisits];
   c->stats.render_ms = ms;
   c->stats.stack_overflows = static_cast<int32_t>(ovf);
   if (ovf) return fail(c, XNH_E_OVERFLOW, "traversal stack overflow");

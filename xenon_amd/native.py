@@ -96,7 +96,8 @@ class Hit(C.Structure):
 class Stats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("node_tests", C.c_uint64), ("interior", C.c_uint64), ("tri_tests", C.c_uint64),
-                ("inst_visits", C.c_uint64), ("nonfinite", C.c_uint64), ("render_ms", C.c_double),
+                ("inst_visits", C.c_uint64), ("shadow_node_tests", C.c_uint64), ("shadow_interior", C.c_uint64),
+                ("shadow_tri_tests", C.c_uint64), ("shadow_inst_visits", C.c_uint64), ("nonfinite", C.c_uint64), ("render_ms", C.c_double),
                 ("closest_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
                 ("closest_launches", C.c_int32), ("stack_overflows", C.c_int32)]
 

@@ -91,6 +91,8 @@ class HipScene:
 
     def blas(self, mesh: int):
         mv = self._flat.meshes[mesh]
+        if mv.num_nodes == 0:
+            return np.zeros(0, NODE_DTYPE), np.zeros(0, np.uint32)
         nodes = np.ctypeslib.as_array(C.cast(mv.nodes, C.POINTER(C.c_uint8)), shape=(mv.num_nodes * 32,))
         nodes = nodes.view(NODE_DTYPE).copy()
         prims = np.ctypeslib.as_array(mv.prims, shape=(mv.num_triangles,)).copy()

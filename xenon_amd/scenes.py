@@ -132,11 +132,18 @@ class SceneDesc:
 # geometry helpers
 # ---------------------------------------------------------------------------
 
-def _quad(verts, tris, p0, p1, p2, p3):
-    """Appends quad p0-p1-p2-p3 (counter-clockwise seen from its front side)
-    as two triangles with their own four vertices (flat shading)."""
+def _quad(verts, tris, p0, p1, p2, p3, facing):
+    """Appends the planar quad p0-p1-p2-p3 (corners in cyclic order) as two
+    triangles with their own four vertices (flat shading), wound so that the
+    geometric normal cross(p1 - p0, p2 - p0) points along `facing`. The
+    reference never face-forwards normals and takes NEE only on the normal's
+    side (render/cpu_tiled.cpp:117-121), so orientation matters."""
+    P = [np.asarray(p, np.float64) for p in (p0, p1, p2, p3)]
+    n = np.cross(P[1] - P[0], P[2] - P[0])
+    if float(np.dot(n, facing)) < 0.0:
+        P = [P[0], P[3], P[2], P[1]]
     base = len(verts)
-    verts.extend([p0, p1, p2, p3])
+    verts.extend([tuple(p) for p in P])
     tris.append((base, base + 1, base + 2))
     tris.append((base, base + 2, base + 3))
 
@@ -153,9 +160,12 @@ def _box(verts, tris, lo, hi, rot_y_deg=0.0, center=None):
         a = math.radians(rot_y_deg)
         ca, sa = math.cos(a), math.sin(a)
         c = [(cx + (x - cx) * ca + (z - cz) * sa, y, cz - (x - cx) * sa + (z - cz) * ca) for x, y, z in c]
+    C = np.asarray(c, np.float64)
+    mid = C.mean(0)
     faces = [(0, 3, 2, 1), (4, 5, 6, 7), (0, 4, 7, 3), (1, 2, 6, 5), (0, 1, 5, 4), (3, 7, 6, 2)]
     for f in faces:
-        _quad(verts, tris, *[c[i] for i in f])
+        fc = C[list(f)].mean(0)
+        _quad(verts, tris, *[c[i] for i in f], facing=fc - mid)
 
 
 def _tri_area(p0, p1, p2) -> float:
@@ -183,16 +193,16 @@ def cornell(width=512, height=512, spp=64, max_bounces=8, min_bounces=3) -> Scen
         mats.extend([mat] * (len(tris) - n0))
 
     WHITE, RED, GREEN, LIGHT = 0, 1, 2, 3
-    add(WHITE, _quad, (0, 0, 0), (0, 0, -S), (S, 0, -S), (S, 0, 0))          # floor, +y
-    add(WHITE, _quad, (0, S, 0), (S, S, 0), (S, S, -S), (0, S, -S))          # ceiling, -y
-    add(WHITE, _quad, (0, 0, -S), (0, S, -S), (S, S, -S), (S, 0, -S))        # back, +z
-    add(RED, _quad, (0, 0, 0), (0, S, 0), (0, S, -S), (0, 0, -S))            # left, +x
-    add(GREEN, _quad, (S, 0, 0), (S, 0, -S), (S, S, -S), (S, S, 0))          # right, -x
+    add(WHITE, _quad, (0, 0, 0), (0, 0, -S), (S, 0, -S), (S, 0, 0), facing=(0, 1, 0))     # floor
+    add(WHITE, _quad, (0, S, 0), (S, S, 0), (S, S, -S), (0, S, -S), facing=(0, -1, 0))    # ceiling
+    add(WHITE, _quad, (0, 0, -S), (0, S, -S), (S, S, -S), (S, 0, -S), facing=(0, 0, 1))   # back
+    add(RED, _quad, (0, 0, 0), (0, S, 0), (0, S, -S), (0, 0, -S), facing=(1, 0, 0))       # left
+    add(GREEN, _quad, (S, 0, 0), (S, 0, -S), (S, S, -S), (S, S, 0), facing=(-1, 0, 0))    # right
     add(WHITE, _box, (1.30, 0.0, -4.72), (2.95, 3.30, -3.07), rot_y_deg=15.0, center=(2.125, -3.895))
     add(WHITE, _box, (3.10, 0.0, -2.50), (4.75, 1.65, -0.85), rot_y_deg=-18.0, center=(3.925, -1.675))
     ly = S - 0.01
-    lq = [(2.13, ly, -2.27), (3.43, ly, -2.27), (3.43, ly, -3.32), (2.13, ly, -3.32)]   # faces -y
-    add(LIGHT, _quad, *lq)
+    lq = [(2.13, ly, -2.27), (3.43, ly, -2.27), (3.43, ly, -3.32), (2.13, ly, -3.32)]
+    add(LIGHT, _quad, *lq, facing=(0, -1, 0))
     nt = len(tris)
     mesh = Mesh(np.array(verts, dtype=F32), np.array(tris, dtype=np.int32), np.array(mats, dtype=np.int32))
     materials = [
@@ -449,3 +459,111 @@ def instancing(width=1920, height=1080, spp=64, max_bounces=10, min_bounces=3, n
 
 
 CONFIGS = {0: cornell, 1: big_mesh, 2: instancing}
+
+
+# ---------------------------------------------------------------------------
+# config 4: caustics through a wall opening
+# ---------------------------------------------------------------------------
+
+def caustic_room(width=1280, height=720, spp=64, freq=158, max_bounces=8, min_bounces=3, seed=7,
+                 emission=(400.0, 370.0, 320.0)) -> SceneDesc:
+    """Closed diffuse room (10 x 6 x 10) lit only by a 0.5 x 0.5 emissive quad
+    at the back of a small alcove behind a 1 x 1 opening in the +x wall; a
+    displaced geodesic sphere of 20*freq^2 triangles (freq 158: 499,280) of
+    delta glass (ior 1.5, roughness 0.02, alpha 4e-4 < 1e-3) stands in the
+    room. The sky is off, so every photon enters through the opening."""
+    verts, tris, mats = [], [], []
+
+    def add(mat, fn, *a, **k):
+        n0 = len(tris)
+        fn(verts, tris, *a, **k)
+        mats.extend([mat] * (len(tris) - n0))
+
+    X, Y, Z = 5.0, 6.0, 5.0
+    WALL, FLOOR, LIGHTM = 0, 1, 2
+    inward = {"+y": (0, 1, 0), "-y": (0, -1, 0), "+z": (0, 0, 1), "-z": (0, 0, -1), "+x": (1, 0, 0),
+              "-x": (-1, 0, 0)}
+    add(FLOOR, _quad, (-X, 0, Z), (X, 0, Z), (X, 0, -Z), (-X, 0, -Z), facing=inward["+y"])    # floor
+    add(WALL, _quad, (-X, Y, Z), (-X, Y, -Z), (X, Y, -Z), (X, Y, Z), facing=inward["-y"])     # ceiling
+    add(WALL, _quad, (-X, 0, -Z), (X, 0, -Z), (X, Y, -Z), (-X, Y, -Z), facing=inward["+z"])   # back
+    add(WALL, _quad, (-X, 0, Z), (-X, Y, Z), (X, Y, Z), (X, 0, Z), facing=inward["-z"])       # front
+    add(WALL, _quad, (-X, 0, Z), (-X, 0, -Z), (-X, Y, -Z), (-X, Y, Z), facing=inward["+x"])   # left
+    # right wall (x = +X, facing -x) with a 1 x 1 opening at y in [3.5, 4.5], z in [-0.5, 0.5]
+    y0, y1, z0, z1 = 3.5, 4.5, -0.5, 0.5
+    add(WALL, _quad, (X, 0, Z), (X, y0, Z), (X, y0, -Z), (X, 0, -Z), facing=inward["-x"])
+    add(WALL, _quad, (X, y1, Z), (X, Y, Z), (X, Y, -Z), (X, y1, -Z), facing=inward["-x"])
+    add(WALL, _quad, (X, y0, Z), (X, y1, Z), (X, y1, z1), (X, y0, z1), facing=inward["-x"])
+    add(WALL, _quad, (X, y0, z0), (X, y1, z0), (X, y1, -Z), (X, y0, -Z), facing=inward["-x"])
+    # alcove behind the opening (depth 0.8), facing inward
+    D = X + 0.8
+    add(WALL, _quad, (X, y0, z1), (D, y0, z1), (D, y0, z0), (X, y0, z0), facing=inward["+y"])
+    add(WALL, _quad, (X, y1, z1), (X, y1, z0), (D, y1, z0), (D, y1, z1), facing=inward["-y"])
+    add(WALL, _quad, (X, y0, z0), (D, y0, z0), (D, y1, z0), (X, y1, z0), facing=inward["+z"])
+    add(WALL, _quad, (X, y0, z1), (X, y1, z1), (D, y1, z1), (D, y0, z1), facing=inward["-z"])
+    add(WALL, _quad, (D, y0, z1), (D, y1, z1), (D, y1, z0), (D, y0, z0), facing=inward["-x"])
+    # emitter: 0.5 x 0.5, just in front of the alcove back, facing -x
+    e = D - 0.01
+    lq = [(e, 3.75, 0.25), (e, 4.25, 0.25), (e, 4.25, -0.25), (e, 3.75, -0.25)]
+    add(LIGHTM, _quad, *lq, facing=inward["-x"])
+    nt = len(tris)
+    room = Mesh(np.array(verts, F32), np.array(tris, np.int32), np.array(mats, np.int32))
+    V, F = geodesic_sphere(freq)
+    P = displaced(V, seed=seed, amplitude=0.05) * 1.3 + np.array([0.3, 1.45, -0.2])
+    glass = Mesh(P.astype(F32), F.astype(np.int32), np.full(F.shape[0], 3, np.int32))
+    materials = [Material(base=(0.75, 0.75, 0.75), roughness=1.0, ior=1.0),
+                 Material(base=(0.60, 0.55, 0.50), roughness=1.0, ior=1.0),
+                 Material(base=(0.8, 0.8, 0.8), roughness=1.0, ior=1.0, emission=emission),
+                 Material(base=(1.0, 1.0, 1.0), roughness=0.02, ior=1.5, transmission=1.0)]
+    Pr = room.positions
+    lights = []
+    for t in (nt - 2, nt - 1):
+        i0, i1, i2 = room.indices[t]
+        lights.append(Light(0, t, tuple(emission), _tri_area(Pr[i0], Pr[i1], Pr[i2])))
+    cam = Camera(eye=(-4.2, 3.2, 4.2), target=(0.4, 1.3, -0.3), fov=58.0)
+    sky = Sky(intensity=0.0)
+    return SceneDesc("caustic_room", [room, glass], materials, lights, cam, width, height, spp,
+                     min_bounces=min_bounces, max_bounces=max_bounces, sky=sky)
+
+
+# ---------------------------------------------------------------------------
+# config 3: batched jobs
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Job:
+    kind: int            # generator: 0 cornell, 1 big mesh, 2 instancing
+    width: int
+    height: int
+    spp: int
+    max_bounces: int
+    seed: int
+
+
+def job_list(n=32, seed=6, res_range=(256, 2048), spp_choices=(16, 32, 64, 128, 256, 512, 1024),
+             depth_range=(4, 12)) -> list:
+    """`n` render jobs drawn by seed: generator 0-2, width and height uniform
+    in `res_range` (multiples of 16), spp from `spp_choices`, max depth
+    uniform in `depth_range`."""
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n):
+        w, h = (int(rng.integers(res_range[0] // 16, res_range[1] // 16 + 1)) * 16 for _ in range(2))
+        jobs.append(Job(int(rng.integers(0, 3)), w, h, int(rng.choice(spp_choices)),
+                        int(rng.integers(depth_range[0], depth_range[1] + 1)), int(rng.integers(0, 1 << 30))))
+    return jobs
+
+
+def job_scene(job: Job, small: bool = False) -> SceneDesc:
+    """The scene of a job; `small` shrinks the geometry (for tests)."""
+    if job.kind == 0:
+        d = cornell(job.width, job.height, job.spp, max_bounces=job.max_bounces)
+    elif job.kind == 1:
+        d = big_mesh(job.width, job.height, job.spp, level=4 if small else 6, max_bounces=job.max_bounces)
+    else:
+        d = instancing(job.width, job.height, job.spp, max_bounces=job.max_bounces,
+                       n_inst=64 if small else 512, tri_range=(500, 2000) if small else (2000, 20000),
+                       n_emit=16)
+    return d
+
+
+CONFIGS[4] = caustic_room
