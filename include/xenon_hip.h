@@ -38,9 +38,13 @@ extern "C" {
 #define XNH_E_ALLOC (-6)       /* device or host allocation failed */
 
 typedef struct xnh_ctx xnh_ctx;
-typedef struct xnh_scene xnh_scene;
+typedef struct xnh_scene xnh_scene;          /* host scene builder */
+typedef struct xnh_gpu_scene xnh_gpu_scene;  /* a scene resident on the device */
 
-/* Renderer configuration. Zero fields take defaults. */
+#define XNH_FLAG_NO_GRAPH 1                  /* launch kernels directly, no hipGraph replay */
+
+/* Renderer configuration. width/height/bounces are used as given; the
+ * zero-valued tuning fields take defaults. */
 typedef struct {
   int32_t width, height;
   int32_t max_bounces;       /* reference default 8 (scene/scene_file.h:16) */
@@ -48,7 +52,7 @@ typedef struct {
   int32_t spp_per_launch;    /* samples per pixel in flight per wavefront batch (0: auto) */
   int32_t block_size;        /* work-group size of the traversal kernels (0: auto) */
   int32_t lds_stack;         /* traversal stack entries held in LDS per lane (0: auto) */
-  int32_t flags;             /* reserved, 0 */
+  int32_t flags;             /* XNH_FLAG_* */
 } xnh_config;
 
 /* Pinhole camera as the reference's Camera exposes it for GPU upload
@@ -188,6 +192,32 @@ int xnh_upload_scene(xnh_ctx* ctx, const xnh_flat_scene* scene);
  * device accumulator. rgb_out (optional, host, W*H*3) receives the running
  * average over all samples added since the last reset. */
 int xnh_render(xnh_ctx* ctx, const xnh_camera* cam, int32_t spp_first, int32_t spp_count, float* rgb_out);
+/* One render job of a batched call: `scene` (NULL: the scene given to
+ * xnh_upload_scene) seen through `camera` at width x height, samples
+ * spp_first .. spp_first+spp_count-1, its own depth limits. Outputs are
+ * optional host buffers of W*H*3 floats in pixel order: the average and/or
+ * the sums of this call's samples. */
+typedef struct {
+  const xnh_gpu_scene* scene;
+  xnh_camera camera;
+  int32_t width, height;
+  int32_t spp_first, spp_count;
+  int32_t max_bounces, min_bounces;
+  float* rgb_out;
+  float* sum_out;
+} xnh_job;
+
+/* Additional device-resident scenes for xnh_render_jobs. No counterpart in
+ * the reference (one scene per renderer there). */
+int xnh_scene_upload(xnh_ctx* ctx, const xnh_flat_scene* scene, xnh_gpu_scene** out);
+void xnh_scene_release(xnh_ctx* ctx, xnh_gpu_scene* scene);
+/* Renders many jobs of uneven resolution, sample count and depth together:
+ * their samples share the wavefront queues (per-job queue segments) and
+ * every batch is one replay of the same captured kernel sequence. Each
+ * job's result equals rendering it alone, bit for bit. No counterpart in
+ * the reference. */
+int xnh_render_jobs(xnh_ctx* ctx, const xnh_job* jobs, int32_t njobs);
+
 /* Copies the raw per-pixel sums (W*H*3) and returns the sample count. */
 int xnh_read_accum(xnh_ctx* ctx, float* sum_out, int32_t* spp_out);
 int xnh_reset(xnh_ctx* ctx);

@@ -7,6 +7,6 @@ and the seeded workload generators in ``scenes``.
 """
 from . import scenes  # noqa: F401
 from .native import XnhError  # noqa: F401
-from .renderer import HIT_DTYPE, NODE_DTYPE, HipRenderer, HipScene  # noqa: F401
+from .renderer import HIT_DTYPE, NODE_DTYPE, GpuScene, HipRenderer, HipScene  # noqa: F401
 
-__all__ = ["HipRenderer", "HipScene", "XnhError", "scenes", "HIT_DTYPE", "NODE_DTYPE"]
+__all__ = ["GpuScene", "HipRenderer", "HipScene", "XnhError", "scenes", "HIT_DTYPE", "NODE_DTYPE"]

@@ -63,9 +63,30 @@ struct DevCamera {
   float eye[3], llc[3], horiz[3], vert[3];
 };
 
-// Wavefront state. Path queues are double-buffered; entry i of a queue is
-// one live path. Per-sample ("slot") buffers are indexed by
-// slot = sample_in_batch * num_pixels + pixel and hold radiance sums.
+// Wavefront state. A render call is a list of jobs (scene, camera,
+// resolution, sample range, depth); the host cuts their samples into
+// batches of chunks, each chunk a contiguous sample range of one job, and
+// uploads all chunk tables once. Chunk c of a batch owns queue entries and
+// per-sample slots [base, base + cap) (cap = pixels * samples) in both path
+// queues and [2 base, 2 base + 2 cap) in the shadow queue: per-job queue
+// segments, so every 64-entry unit of work belongs to exactly one job and a
+// wave reads its job's scene through scalar loads. Slot order inside a chunk
+// is sample-major: slot = base + s * pixels + pixel.
+constexpr int kMaxChunks = 64;
+
+struct DevJob {
+  DevScene S;
+  DevCamera cam;
+  int32_t W, H, minb, maxb;
+  float4* accum;    // per-pixel sums of this job
+  int32_t pad[2];
+};
+
+struct DevChunk {
+  int32_t job, spp_first, nsamples, npix;
+  int32_t base, cap, pad0, pad1;
+};
+
 struct PathQueue {
   float4* ray0;     // {o.xyz, d.x}
   float4* ray1;     // {d.y, d.z, prev_pdf, flags}
@@ -75,27 +96,30 @@ struct PathQueue {
 
 struct WaveBuffers {
   PathQueue q[2];
-  float4* hit;      // {t, u, v, instance index (bits), } per queue entry; inst -1 = miss
+  float4* hit;      // {t, u, v, instance index (bits)} per queue entry; -1 = miss
   uint32_t* hit_slot;  // global triangle slot per queue entry
   float4* sh0;      // shadow ray {o.xyz, d.x}
   float4* sh1;      // {d.y, d.z, tmax, slot*2+k}
   float4* sh_c;     // unoccluded contribution
   float4* l_term;   // per slot: emission / sky at path end
   float4* l_nee;    // per slot x 2: area and sun NEE sums
-  float4* accum;    // per pixel running sum
-  uint32_t* counters;   // see Ctr
+  const DevJob* jobs;
+  const DevChunk* chunks;   // [batch][kMaxChunks]
+  const int32_t* nchunks;   // chunks per batch
+  uint32_t* counters;       // see Ctr
   unsigned long long* stats;  // see Stat
   uint32_t* spill;  // traversal stack entries beyond the LDS part
   int32_t capacity;
 };
 
 enum Ctr : int {
-  kCount0 = 0,        // live paths in queue 0
-  kCount1 = 1,        // live paths in queue 1
-  kShadowCount = 2,
-  kOverflow = 3,      // lanes whose stack exceeded kMaxStack
-  kCursorBase = 8,    // dynamic work cursors, one per launch of a batch
-  kNumCtr = 8 + 4 * 64
+  kBatch = 0,                            // batch being rendered (advanced on the device)
+  kOverflow = 1,                         // lanes whose stack exceeded kMaxStack
+  kCnt = 16,                             // [queue][chunk] live paths
+  kShCnt = kCnt + 2 * kMaxChunks,        // [chunk] shadow rays
+  kCursor = kShCnt + kMaxChunks,         // per-launch work cursors of a batch
+  kNumCursors = 2 * 64 + 4,
+  kNumCtr = kCursor + kNumCursors
 };
 
 enum Stat : int {

@@ -133,35 +133,107 @@ __device__ __forceinline__ f3 sky_eval(const DevSky& k, f3 d) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// raygen: per-(pixel, sample) seed and jittered pinhole ray
-// (cpu_tiled.cpp:20-29, camera/camera.h:35-37).
+// Batch bookkeeping shared by the wavefront kernels.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_raygen(WaveBuffers B, DevCamera cam, int W, int H, int spp_first,
-                                                int nsamples, int ncursors) {
-  const int npix = W * H;
-  const int C = npix * nsamples;
-  const int gid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (gid == 0) {
-    B.counters[kCount0] = static_cast<uint32_t>(C);
-    B.counters[kCount1] = 0u;
-    B.counters[kShadowCount] = 0u;
+namespace {
+
+// Per-block copy of the current batch's chunk table with prefix sums of
+// 64-entry work units (traversal / shading) or of pixels (accumulation).
+struct BatchView {
+  int batch, nch;
+  DevChunk ch[kMaxChunks];
+  uint32_t cnt[kMaxChunks];
+  uint32_t prefix[kMaxChunks + 1];
+};
+
+// Loads the batch and builds prefix[] from per-chunk counts (counts may be
+// null: prefix over pixels). Block-wide; ends with a barrier.
+__device__ __forceinline__ void load_batch(BatchView& v, const WaveBuffers& B, const uint32_t* counts,
+                                           bool pixels) {
+  if (threadIdx.x == 0) {
+    v.batch = static_cast<int>(B.counters[kBatch]);
+    v.nch = B.nchunks[v.batch];
   }
-  if (gid < ncursors) B.counters[kCursorBase + gid] = 0u;
-  if (gid >= C) return;
-  const int s = gid / npix;
-  const int pixel = gid - s * npix;
-  const int spp = spp_first + s;
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < v.nch) {
+    v.ch[threadIdx.x] = B.chunks[v.batch * kMaxChunks + threadIdx.x];
+    v.cnt[threadIdx.x] = counts ? counts[threadIdx.x] : 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    v.prefix[0] = 0;
+    for (int c = 0; c < v.nch; ++c) {
+      acc += pixels ? static_cast<uint32_t>(v.ch[c].npix) : ((v.cnt[c] + 63u) >> 6);
+      v.prefix[c + 1] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// Last chunk c with prefix[c] <= u: the (non-empty) chunk holding item u.
+__device__ __forceinline__ int find_chunk(const BatchView& v, uint32_t u) {
+  int lo = 0, hi = v.nch;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (v.prefix[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// raygen: per-(pixel, sample) seed and jittered pinhole ray
+// (cpu_tiled.cpp:20-29, camera/camera.h:35-37); resets the batch's queue
+// counters and work cursors.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
+  __shared__ BatchView v;
+  if (threadIdx.x == 0) {
+    v.batch = static_cast<int>(B.counters[kBatch]);
+    v.nch = B.nchunks[v.batch];
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < v.nch) v.ch[threadIdx.x] = B.chunks[v.batch * kMaxChunks + threadIdx.x];
+  __syncthreads();
+  const int gid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (gid < v.nch) {
+    const DevChunk& ck = v.ch[gid];
+    B.counters[kCnt + gid] = B.jobs[ck.job].maxb > 0 ? static_cast<uint32_t>(ck.cap) : 0u;
+    B.counters[kCnt + kMaxChunks + gid] = 0u;
+    B.counters[kShCnt + gid] = 0u;
+  }
+  if (gid < kNumCursors) B.counters[kCursor + gid] = 0u;
+  if (v.nch == 0) return;
+  const DevChunk& last = v.ch[v.nch - 1];
+  if (gid >= last.base + last.cap) return;
+  int lo = 0, hi = v.nch;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (v.ch[mid].base <= gid) lo = mid;
+    else hi = mid;
+  }
+  const DevChunk& ck = v.ch[lo];
+  const DevJob& J = B.jobs[ck.job];
+  const int local = gid - ck.base;
+  const int s = local / ck.npix;
+  const int pixel = local - s * ck.npix;
+  const int spp = ck.spp_first + s;
   const uint64_t seed = static_cast<uint64_t>(pixel) + static_cast<uint64_t>(spp) * 0xdeadbeefULL;
   Rng rng = rng_for(seed);
-  const int x = pixel % W, y = pixel / W;
+  const int x = pixel % J.W, y = pixel / J.W;
   const float fx = static_cast<float>(x) + rng.f();
   const float fy = static_cast<float>(y) + rng.f();
-  const float u = fx / static_cast<float>(W), v = fy / static_cast<float>(H);
-  const f3 eye{cam.eye[0], cam.eye[1], cam.eye[2]};
-  const f3 llc{cam.llc[0], cam.llc[1], cam.llc[2]};
-  const f3 hz{cam.horiz[0], cam.horiz[1], cam.horiz[2]};
-  const f3 vt{cam.vert[0], cam.vert[1], cam.vert[2]};
-  const f3 d = normalize(llc + u * hz + v * vt - eye);
+  const float u = fx / static_cast<float>(J.W), w = fy / static_cast<float>(J.H);
+  const f3 eye{J.cam.eye[0], J.cam.eye[1], J.cam.eye[2]};
+  const f3 llc{J.cam.llc[0], J.cam.llc[1], J.cam.llc[2]};
+  const f3 hz{J.cam.horiz[0], J.cam.horiz[1], J.cam.horiz[2]};
+  const f3 vt{J.cam.vert[0], J.cam.vert[1], J.cam.vert[2]};
+  const f3 d = normalize(llc + u * hz + w * vt - eye);
   const PathQueue& q = B.q[0];
   q.ray0[gid] = make_float4(eye.x, eye.y, eye.z, d.x);
   q.ray1[gid] = make_float4(d.y, d.z, 0.f, __uint_as_float(1u));
@@ -174,27 +246,36 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B, DevCamera cam, in
 }
 
 // ---------------------------------------------------------------------------
-// closest: persistent closest-hit traversal of queue qi.
+// closest: persistent closest-hit traversal of queue qi. Each wave takes
+// 64-entry units from a device cursor; a unit lies in one chunk, so the
+// job (and its scene) is wave-uniform.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_closest(DevScene S, WaveBuffers B, int qi, int cursor, int lds_depth) {
+__global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int cursor, int lds_depth) {
   extern __shared__ uint32_t lds[];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    B.counters[kCount0 + (1 - qi)] = 0u;
-    B.counters[kShadowCount] = 0u;
+  __shared__ BatchView v;
+  const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
+  load_batch(v, B, counts, false);
+  if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) < v.nch) {
+    B.counters[kCnt + (1 - qi) * kMaxChunks + threadIdx.x] = 0u;
+    B.counters[kShCnt + threadIdx.x] = 0u;
   }
-  const uint32_t count = B.counters[kCount0 + qi];
+  const uint32_t total = v.prefix[v.nch];
   const PathQueue q = B.q[qi];
   TravStack st = make_stack(lds, B.spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
   for (;;) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(&B.counters[kCursorBase + cursor], 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    const uint32_t i = base + static_cast<uint32_t>(lane_id());
-    if (i < count) {
+    uint32_t unit = 0;
+    if (lane_id() == 0) unit = atomicAdd(&B.counters[kCursor + cursor], 1u);
+    unit = static_cast<uint32_t>(uniform(static_cast<int>(__shfl(unit, 0))));
+    if (unit >= total) break;
+    const int ci = uniform(find_chunk(v, unit));
+    const DevChunk& ck = v.ch[ci];
+    const uint32_t off = (unit - v.prefix[ci]) * 64u + static_cast<uint32_t>(lane_id());
+    if (off < v.cnt[ci]) {
+      const DevScene& S = B.jobs[uniform(ck.job)].S;
+      const uint32_t i = static_cast<uint32_t>(ck.base) + off;
       const float4 r0 = q.ray0[i], r1 = q.ray1[i];
       const f3 o{r0.x, r0.y, r0.z}, d{r0.w, r1.x, r1.y};
       HitRef h{0.f, 0.f, 0.f, 0u, -1};
@@ -213,15 +294,23 @@ __global__ void __launch_bounds__(256) k_closest(DevScene S, WaveBuffers B, int 
 // ---------------------------------------------------------------------------
 // shade: emission + MIS, area and sun NEE, Russian roulette, BSDF sampling
 // (cpu_tiled.cpp:30-216) for every entry of queue qi at depth `depth`.
+// Survivors and shadow rays are appended to the same chunk's segments.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_shade(DevScene S, WaveBuffers B, int qi, int depth, int min_b, int max_b) {
-  const uint32_t count = B.counters[kCount0 + qi];
+__global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth) {
+  __shared__ BatchView v;
+  const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
+  load_batch(v, B, counts, false);
+  const uint32_t total = v.prefix[v.nch];
   const PathQueue q = B.q[qi];
   const PathQueue qn = B.q[1 - qi];
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t wave_base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  for (uint32_t base = wave_base0; base < count; base += stride) {
-    const uint32_t i = base + static_cast<uint32_t>(lane_id());
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t unit = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); unit < total; unit += waves) {
+    const int ci = uniform(find_chunk(v, unit));
+    const DevChunk& ck = v.ch[ci];
+    const DevJob& J = B.jobs[uniform(ck.job)];
+    const DevScene& S = J.S;
+    const uint32_t off = (unit - v.prefix[ci]) * 64u + static_cast<uint32_t>(lane_id());
+    const uint32_t i = static_cast<uint32_t>(ck.base) + off;
     bool want_a = false, want_s = false, want_n = false;
     f3 ao{}, ad{}, ac{}, so{}, sd{}, sc{};
     float atmax = 0.f;
@@ -230,7 +319,7 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, WaveBuffers B, int qi
     uint32_t ndelta = 0u;
     uint32_t slot = 0u;
     Rng rng{0};
-    if (i < count) {
+    if (off < v.cnt[ci]) {
       const float4 r0 = q.ray0[i], r1 = q.ray1[i], th = q.thr[i];
       const uint4 ps = q.st[i];
       const f3 o{r0.x, r0.y, r0.z}, d{r0.w, r1.x, r1.y};
@@ -341,9 +430,9 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, WaveBuffers B, int qi
           }
           // The last segment's continuation is never traced: its roulette
           // and BSDF draws cannot change the radiance, so they are skipped.
-          if (depth < max_b - 1) {
+          if (depth < J.maxb - 1) {
             bool alive = true;
-            if (depth >= min_b) {   // Russian roulette (cpu_tiled.cpp:189-194)
+            if (depth >= J.minb) {   // Russian roulette (cpu_tiled.cpp:189-194)
               const float pm = smax(0.05f, smax(smax(T.x, T.y), T.z));
               if (rng.f() > pm) alive = false;
               else T = T * (1.f / pm);
@@ -357,9 +446,9 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, WaveBuffers B, int qi
               } else if (bs.delta) {
                 T = T * bs.f;
               } else {
-                const float ci = fabsf(bs.wi.z);
-                if (bs.pdf < 1e-8f || ci < 1e-8f) alive = false;
-                else T = T * (bs.f * ci / bs.pdf);
+                const float ci2 = fabsf(bs.wi.z);
+                if (bs.pdf < 1e-8f || ci2 < 1e-8f) alive = false;
+                else T = T * (bs.f * ci2 / bs.pdf);
               }
               if (alive) {
                 const f3 wiw = fs.world(bs.wi);
@@ -375,10 +464,12 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, WaveBuffers B, int qi
         }
       }
     }
-    // convergent section: compacted appends
-    const uint32_t ia = wave_append(&B.counters[kShadowCount], want_a);
-    const uint32_t is = wave_append(&B.counters[kShadowCount], want_s);
-    const uint32_t in = wave_append(&B.counters[kCount0 + (1 - qi)], want_n);
+    // convergent section: compacted appends into this chunk's segments
+    const uint32_t sbase = 2u * static_cast<uint32_t>(ck.base);
+    const uint32_t ia = sbase + wave_append(&B.counters[kShCnt + ci], want_a);
+    const uint32_t is = sbase + wave_append(&B.counters[kShCnt + ci], want_s);
+    const uint32_t in = static_cast<uint32_t>(ck.base) +
+                        wave_append(&B.counters[kCnt + (1 - qi) * kMaxChunks + ci], want_n);
     if (want_a) {
       B.sh0[ia] = make_float4(ao.x, ao.y, ao.z, ad.x);
       B.sh1[ia] = make_float4(ad.y, ad.z, atmax, __uint_as_float(slot * 2u));
@@ -402,20 +493,27 @@ __global__ void __launch_bounds__(256) k_shade(DevScene S, WaveBuffers B, int qi
 // shadow: persistent any-hit traversal; unoccluded contributions are added
 // to their path's per-slot NEE sum (one writer per (slot, light kind)).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_shadow(DevScene S, WaveBuffers B, int cursor, int lds_depth) {
+__global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int lds_depth) {
   extern __shared__ uint32_t lds[];
-  const uint32_t count = B.counters[kShadowCount];
+  __shared__ BatchView v;
+  const uint32_t* counts = B.counters + kShCnt;
+  load_batch(v, B, counts, false);
+  const uint32_t total = v.prefix[v.nch];
   TravStack st = make_stack(lds, B.spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
   for (;;) {
-    uint32_t base = 0;
-    if (lane_id() == 0) base = atomicAdd(&B.counters[kCursorBase + cursor], 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    const uint32_t i = base + static_cast<uint32_t>(lane_id());
-    if (i < count) {
+    uint32_t unit = 0;
+    if (lane_id() == 0) unit = atomicAdd(&B.counters[kCursor + cursor], 1u);
+    unit = static_cast<uint32_t>(uniform(static_cast<int>(__shfl(unit, 0))));
+    if (unit >= total) break;
+    const int ci = uniform(find_chunk(v, unit));
+    const DevChunk& ck = v.ch[ci];
+    const uint32_t off = (unit - v.prefix[ci]) * 64u + static_cast<uint32_t>(lane_id());
+    if (off < v.cnt[ci]) {
+      const DevScene& S = B.jobs[uniform(ck.job)].S;
+      const uint32_t i = 2u * static_cast<uint32_t>(ck.base) + off;
       const float4 r0 = B.sh0[i], r1 = B.sh1[i];
       const f3 o{r0.x, r0.y, r0.z}, d{r0.w, r1.x, r1.y};
       HitRef h{0.f, 0.f, 0.f, 0u, -1};
@@ -439,17 +537,24 @@ __global__ void __launch_bounds__(256) k_shadow(DevScene S, WaveBuffers B, int c
 }
 
 // ---------------------------------------------------------------------------
-// accumulate: per pixel, add each sample's path radiance in sample order
-// (cpu_tiled.cpp:218). NaN/Inf are kept, as the reference keeps them, and
-// counted.
+// accumulate: per (chunk, pixel), add each sample's path radiance to the
+// job's sums in sample order (cpu_tiled.cpp:218). NaN/Inf are kept, as the
+// reference keeps them, and counted.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_accumulate(WaveBuffers B, int npix, int nsamples) {
-  const int p = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void __launch_bounds__(256) k_accumulate(WaveBuffers B) {
+  __shared__ BatchView v;
+  load_batch(v, B, nullptr, true);
+  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long bad = 0;
-  if (p < npix) {
-    float4 a = B.accum[p];
-    for (int s = 0; s < nsamples; ++s) {
-      const int slot = s * npix + p;
+  unsigned long long paths = 0;
+  if (gid < v.prefix[v.nch]) {
+    const int ci = find_chunk(v, gid);
+    const DevChunk& ck = v.ch[ci];
+    const int p = static_cast<int>(gid - v.prefix[ci]);
+    float4* accum = B.jobs[ck.job].accum;
+    float4 a = accum[p];
+    for (int s = 0; s < ck.nsamples; ++s) {
+      const int slot = ck.base + s * ck.npix + p;
       const float4 t = B.l_term[slot], n0 = B.l_nee[2 * slot], n1 = B.l_nee[2 * slot + 1];
       const f3 L = f3{t.x, t.y, t.z} + f3{n0.x, n0.y, n0.z} + f3{n1.x, n1.y, n1.z};
       if (!finite3(L)) ++bad;
@@ -457,10 +562,19 @@ __global__ void __launch_bounds__(256) k_accumulate(WaveBuffers B, int npix, int
       a.y = a.y + L.y;
       a.z = a.z + L.z;
     }
-    B.accum[p] = a;
+    accum[p] = a;
+    paths = static_cast<unsigned long long>(ck.nsamples);
   }
-  const unsigned long long tot = wave_sum(bad);
-  if (lane_id() == 0 && tot) atomicAdd(&B.stats[kStNonFinite], tot);
+  const unsigned long long tot = wave_sum(bad), np = wave_sum(paths);
+  if (lane_id() == 0) {
+    if (tot) atomicAdd(&B.stats[kStNonFinite], tot);
+    if (np) atomicAdd(&B.stats[kStPaths], np);
+  }
+}
+
+// Moves the device to the next batch; the last node of a batch.
+__global__ void k_advance(uint32_t* counters) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) counters[kBatch] = counters[kBatch] + 1u;
 }
 
 __global__ void __launch_bounds__(256) k_average(const float4* accum, float* out, int npix, float inv_spp) {
@@ -560,36 +674,38 @@ int num_cus() {
   return prop.multiProcessorCount;
 }
 
-hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, const DevCamera& cam, int W, int H, int spp_first,
-                         int nsamples, int ncursors) {
-  const int C = W * H * nsamples;
-  const int n = C > ncursors ? C : ncursors;
-  hipLaunchKernelGGL(k_raygen, dim3((n + 255) / 256), dim3(256), 0, s, B, cam, W, H, spp_first, nsamples, ncursors);
+hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths) {
+  const int n = max_paths > kNumCursors ? max_paths : kNumCursors;
+  hipLaunchKernelGGL(k_raygen, dim3((n + 255) / 256), dim3(256), 0, s, B);
   return hipGetLastError();
 }
 
-hipError_t launch_closest(hipStream_t s, const DevScene& S, const WaveBuffers& B, int qi, int cursor, int grid,
-                          int block, int lds_depth) {
-  hipLaunchKernelGGL(k_closest, dim3(grid), dim3(block), static_cast<size_t>(block) * lds_depth * 4, s, S, B, qi,
-                     cursor, lds_depth);
-  return hipGetLastError();
-}
-
-hipError_t launch_shade(hipStream_t s, const DevScene& S, const WaveBuffers& B, int qi, int depth, int min_b,
-                        int max_b, int grid) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, S, B, qi, depth, min_b, max_b);
-  return hipGetLastError();
-}
-
-hipError_t launch_shadow(hipStream_t s, const DevScene& S, const WaveBuffers& B, int cursor, int grid, int block,
-                         int lds_depth) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(block), static_cast<size_t>(block) * lds_depth * 4, s, S, B, cursor,
+hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block,
+                          int lds_depth) {
+  hipLaunchKernelGGL(k_closest, dim3(grid), dim3(block), static_cast<size_t>(block) * lds_depth * 4, s, B, qi, cursor,
                      lds_depth);
   return hipGetLastError();
 }
 
-hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int npix, int nsamples) {
-  hipLaunchKernelGGL(k_accumulate, dim3((npix + 255) / 256), dim3(256), 0, s, B, npix, nsamples);
+hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, int grid) {
+  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, B, qi, depth);
+  return hipGetLastError();
+}
+
+hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth) {
+  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(block), static_cast<size_t>(block) * lds_depth * 4, s, B, cursor,
+                     lds_depth);
+  return hipGetLastError();
+}
+
+hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int max_pixels) {
+  const int n = max_pixels > 1 ? max_pixels : 1;
+  hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(hipStream_t s, uint32_t* counters) {
+  hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, s, counters);
   return hipGetLastError();
 }
 

@@ -10,15 +10,13 @@ namespace xnh {
 int trace_grid(int block, int lds_depth);
 int num_cus();
 
-hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, const DevCamera& cam, int W, int H, int spp_first,
-                         int nsamples, int ncursors);
-hipError_t launch_closest(hipStream_t s, const DevScene& S, const WaveBuffers& B, int qi, int cursor, int grid,
-                          int block, int lds_depth);
-hipError_t launch_shade(hipStream_t s, const DevScene& S, const WaveBuffers& B, int qi, int depth, int min_b,
-                        int max_b, int grid);
-hipError_t launch_shadow(hipStream_t s, const DevScene& S, const WaveBuffers& B, int cursor, int grid, int block,
-                         int lds_depth);
-hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int npix, int nsamples);
+hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths);
+hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block,
+                          int lds_depth);
+hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, int grid);
+hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth);
+hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int max_pixels);
+hipError_t launch_advance(hipStream_t s, uint32_t* counters);
 hipError_t launch_average(hipStream_t s, const float4* accum, float* out, int npix, float inv_spp);
 hipError_t launch_api_closest(hipStream_t s, const DevScene& S, const float4* rays, float* hits, int n,
                               uint32_t* spill, int lds_depth, unsigned long long* stats, uint32_t* overflow_ctr);

@@ -1,11 +1,19 @@
-// api.cpp — the C ABI of libxenon_hip.so (include/xenon_hip.h).
+// api.cpp — renderer half of the C ABI of libxenon_hip.so
+// (include/xenon_hip.h).
 //
-// Host half of the backend: device selection, buffer ownership, scene
-// flattening into the HBM layout of common/dev_types.h, and the bounce loop
-// that enqueues the wavefront kernels on one HIP stream. The loop never
-// reads a count back: every kernel finds its work count in device memory,
-// so a call enqueues 2 + 3*max_bounces launches per batch and synchronises
-// once at the end.
+// Device selection, buffer ownership, scene flattening into the HBM layout
+// of common/dev_types.h, batch scheduling and the bounce loop. A render call
+// is a list of jobs; their samples are cut into batches of chunks (one
+// contiguous sample range of one job per chunk), all chunk tables are
+// uploaded once, and each batch runs
+//
+//   raygen -> [closest(d) -> shade(d) -> shadow(d)] d < max depth -> accumulate -> advance
+//
+// with no host read between kernels: every work count and the batch index
+// live in device memory, so the sequence is captured once as a hipGraph and
+// replayed per batch. With per-kernel profiling on, the same sequence is
+// launched directly with HIP events around the traversal and shading
+// kernels.
 //
 // The reference's CUDA backend aborts on any CUDA error
 // (cuda/cuda_utils.cuh:12-20) and reads counters back ~6 times per bounce
@@ -22,13 +30,14 @@
 
 #include "../common/dev_types.h"
 #include "../device/launch.h"
-#include "scene.h"
 #include "xenon_hip.h"
 
 using namespace xnh;
 
-struct xnh_scene {
-  HostScene s;
+struct xnh_gpu_scene {
+  std::vector<void*> allocs;
+  DevScene S{};
+  size_t bytes = 0;
 };
 
 struct xnh_ctx {
@@ -38,12 +47,21 @@ struct xnh_ctx {
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   std::vector<hipEvent_t> prof_events;
   bool profiling = false;
-  std::vector<void*> scene_allocs;
-  DevScene S{};
+  bool use_graph = true;
+  xnh_gpu_scene own;              // the scene of xnh_upload_scene
   bool has_scene = false;
   WaveBuffers B{};
   std::vector<void*> wave_allocs;
-  int nsamples = 1;        // samples per pixel per batch
+  float4* accum = nullptr;        // the contract's running per-pixel sums
+  DevJob* d_jobs = nullptr;
+  DevChunk* d_chunks = nullptr;
+  int32_t* d_nchunks = nullptr;
+  size_t jobs_cap = 0, batches_cap = 0;
+  uint64_t table_gen = 0;         // bumps when a table is reallocated
+  hipGraphExec_t graph = nullptr;
+  int graph_depth = -1;
+  uint64_t graph_gen = 0;
+  int nsamples = 1;               // samples per pixel per batch at the configured resolution
   int block = 256;
   int lds_depth = 32;
   int grid_trace = 0;
@@ -123,110 +141,8 @@ void face_normal(const float p0[3], const float p1[3], const float p2[3], float 
   out[2] = c[2] * r;
 }
 
-int alloc_waves(xnh_ctx* c) {
-  free_all(c->wave_allocs);
-  const int npix = c->cfg.width * c->cfg.height;
-  const size_t cap = static_cast<size_t>(npix) * c->nsamples;
-  WaveBuffers& B = c->B;
-  B = WaveBuffers{};
-  B.capacity = static_cast<int32_t>(cap);
-  int rc = XNH_OK;
-  for (int k = 0; k < 2 && rc == XNH_OK; ++k) {
-    rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray0, cap);
-    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray1, cap);
-    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].thr, cap);
-    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].st, cap);
-  }
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit, cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit_slot, cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh0, 2 * cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh1, 2 * cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh_c, 2 * cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.l_term, cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.l_nee, 2 * cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.accum, static_cast<size_t>(npix));
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.counters, static_cast<size_t>(kNumCtr));
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.stats, static_cast<size_t>(kNumStat));
-  if (rc == XNH_OK) {
-    const size_t spill = static_cast<size_t>(kMaxStack - c->lds_depth) * c->grid_trace * c->block;
-    rc = dev_alloc(c, c->wave_allocs, &B.spill, spill);
-  }
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &c->d_out, static_cast<size_t>(npix) * 3);
-  if (rc != XNH_OK) return rc;
-  HIP_OR_FAIL(c, hipMemset(B.accum, 0, sizeof(float4) * npix));
-  HIP_OR_FAIL(c, hipMemset(B.counters, 0, sizeof(uint32_t) * kNumCtr));
-  HIP_OR_FAIL(c, hipMemset(B.stats, 0, sizeof(unsigned long long) * kNumStat));
-  return XNH_OK;
-}
-
-}  // namespace
-
-extern "C" {
-
-int xnh_create(const xnh_config* cfg, xnh_ctx** out) {
-  if (!cfg || !out) return XNH_E_INVALID;
-  *out = nullptr;
-  if (cfg->width <= 0 || cfg->height <= 0 || cfg->max_bounces < 0 || cfg->min_bounces < 0) return XNH_E_INVALID;
-  if (cfg->max_bounces > 64) return XNH_E_INVALID;
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return XNH_E_NO_DEVICE;
-  xnh_ctx* c = new (std::nothrow) xnh_ctx();
-  if (!c) return XNH_E_ALLOC;
-  c->cfg = *cfg;
-  c->block = cfg->block_size > 0 ? cfg->block_size : 256;
-  if (c->block % 64 != 0 || c->block > 256) {
-    delete c;
-    return XNH_E_INVALID;
-  }
-  c->lds_depth = cfg->lds_stack > 0 ? std::min(cfg->lds_stack, kMaxStack) : 32;
-  const int npix = cfg->width * cfg->height;
-  if (cfg->spp_per_launch > 0) {
-    c->nsamples = cfg->spp_per_launch;
-  } else {
-    const int target = 8 << 20;   // paths in flight per batch
-    c->nsamples = std::max(1, std::min(64, target / npix));
-  }
-  if (static_cast<long long>(npix) * c->nsamples > (1LL << 30)) {
-    delete c;
-    return XNH_E_INVALID;
-  }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
-    xnh_destroy(c);
-    return XNH_E_HIP;
-  }
-  c->grid_trace = trace_grid(c->block, c->lds_depth);
-  const int cus = num_cus();
-  if (c->grid_trace <= 0 || cus <= 0) {
-    xnh_destroy(c);
-    return XNH_E_HIP;
-  }
-  c->grid_shade = cus * 8;
-  const int rc = alloc_waves(c);
-  if (rc != XNH_OK) {
-    xnh_destroy(c);
-    return rc;
-  }
-  *out = c;
-  return XNH_OK;
-}
-
-void xnh_destroy(xnh_ctx* c) {
-  if (!c) return;
-  if (c->stream) hipStreamSynchronize(c->stream);
-  free_all(c->scene_allocs);
-  free_all(c->wave_allocs);
-  for (hipEvent_t e : c->prof_events) hipEventDestroy(e);
-  if (c->ev_begin) hipEventDestroy(c->ev_begin);
-  if (c->ev_end) hipEventDestroy(c->ev_end);
-  if (c->stream) hipStreamDestroy(c->stream);
-  delete c;
-}
-
-const char* xnh_last_error(const xnh_ctx* c) { return c ? c->err.c_str() : "null context"; }
-
-int xnh_upload_scene(xnh_ctx* c, const xnh_flat_scene* fs) {
-  if (!c || !fs) return XNH_E_INVALID;
+int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
+  if (!c || !fs || !g) return XNH_E_INVALID;
   if (fs->num_meshes < 0 || fs->num_instances < 0 || fs->num_materials < 0 || fs->num_lights < 0 ||
       fs->num_dir_lights < 0 || fs->num_tlas_nodes < 0)
     return fail(c, XNH_E_INVALID, "negative count in flat scene");
@@ -377,26 +293,25 @@ int xnh_upload_scene(xnh_ctx* c, const xnh_flat_scene* fs) {
     suns[2 * i + 1] = f4(d.emission[0], d.emission[1], d.emission[2], 0.f);
   }
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
-  free_all(c->scene_allocs);
-  c->has_scene = false;
+  free_all(g->allocs);
   DevScene S{};
   float4 *p_bn, *p_tr, *p_tn, *p_in, *p_vn, *p_li, *p_ma, *p_su;
   uint32_t* p_to;
   int32_t *p_vi, *p_tm, *p_lm;
-  int rc = dev_upload(c, c->scene_allocs, &p_bn, bnodes);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_tr, tris);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_tn, tnodes);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_to, torder);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_in, inst);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_vi, vidx);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_vn, vnrm);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_tm, tmat);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_lm, lmap);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_li, lights);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_ma, mats);
-  if (rc == XNH_OK) rc = dev_upload(c, c->scene_allocs, &p_su, suns);
+  int rc = dev_upload(c, g->allocs, &p_bn, bnodes);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_tr, tris);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_tn, tnodes);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_to, torder);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_in, inst);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_vi, vidx);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_vn, vnrm);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_tm, tmat);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_lm, lmap);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_li, lights);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_ma, mats);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_su, suns);
   if (rc != XNH_OK) {
-    free_all(c->scene_allocs);
+    free_all(g->allocs);
     return rc;
   }
   S.bnodes = p_bn;
@@ -421,69 +336,226 @@ int xnh_upload_scene(xnh_ctx* c, const xnh_flat_scene* fs) {
   }
   S.sky.sharpness = fs->sky.sharpness;
   S.sky.intensity = fs->sky.intensity;
-  c->S = S;
-  c->has_scene = true;
+  g->S = S;
+  g->bytes = 0;
+  for (const auto* v : {&bnodes, &tris, &tnodes, &inst, &vnrm, &lights, &mats, &suns}) g->bytes += v->size() * 16;
+  g->bytes += (torder.size() + vidx.size() + tmat.size() + lmap.size()) * 4;
   return XNH_OK;
 }
 
-int xnh_set_profiling(xnh_ctx* c, int32_t on) {
-  if (!c) return XNH_E_INVALID;
-  c->profiling = on != 0;
+int alloc_waves(xnh_ctx* c) {
+  free_all(c->wave_allocs);
+  const int npix = c->cfg.width * c->cfg.height;
+  const size_t cap = static_cast<size_t>(npix) * c->nsamples;
+  WaveBuffers& B = c->B;
+  B = WaveBuffers{};
+  B.capacity = static_cast<int32_t>(cap);
+  int rc = XNH_OK;
+  for (int k = 0; k < 2 && rc == XNH_OK; ++k) {
+    rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray0, cap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray1, cap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].thr, cap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].st, cap);
+  }
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit, cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit_slot, cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh0, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh1, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh_c, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.l_term, cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.l_nee, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.counters, static_cast<size_t>(kNumCtr));
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.stats, static_cast<size_t>(kNumStat));
+  if (rc == XNH_OK) {
+    const size_t spill = static_cast<size_t>(kMaxStack - c->lds_depth) * c->grid_trace * c->block;
+    rc = dev_alloc(c, c->wave_allocs, &B.spill, spill);
+  }
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &c->accum, static_cast<size_t>(npix));
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &c->d_out, static_cast<size_t>(npix) * 3);
+  if (rc != XNH_OK) return rc;
+  HIP_OR_FAIL(c, hipMemset(c->accum, 0, sizeof(float4) * npix));
+  HIP_OR_FAIL(c, hipMemset(B.counters, 0, sizeof(uint32_t) * kNumCtr));
+  HIP_OR_FAIL(c, hipMemset(B.stats, 0, sizeof(unsigned long long) * kNumStat));
   return XNH_OK;
 }
 
-int xnh_render(xnh_ctx* c, const xnh_camera* cam, int32_t spp_first, int32_t spp_count, float* rgb_out) {
-  if (!c || !cam || spp_count < 0 || spp_first < 0) return XNH_E_INVALID;
-  if (!c->has_scene) return fail(c, XNH_E_NO_SCENE, "xnh_render before xnh_upload_scene");
-  const int W = c->cfg.width, H = c->cfg.height, npix = W * H;
-  const int maxb = c->cfg.max_bounces, minb = c->cfg.min_bounces;
-  DevCamera dc;
-  for (int k = 0; k < 3; ++k) {
-    dc.eye[k] = cam->eye[k];
-    dc.llc[k] = cam->lower_left[k];
-    dc.horiz[k] = cam->horizontal[k];
-    dc.vert[k] = cam->vertical[k];
+// Grows the job / chunk tables; a reallocation invalidates the cached graph
+// (its kernel arguments hold the table addresses).
+int ensure_tables(xnh_ctx* c, size_t njobs, size_t nbatches) {
+  if (njobs > c->jobs_cap) {
+    if (c->d_jobs) hipFree(c->d_jobs);
+    c->d_jobs = nullptr;
+    size_t cap = std::max<size_t>(njobs, 64);
+    void* p = nullptr;
+    if (hipMalloc(&p, cap * sizeof(DevJob)) != hipSuccess) return fail(c, XNH_E_ALLOC, "job table");
+    c->d_jobs = static_cast<DevJob*>(p);
+    c->jobs_cap = cap;
+    c->table_gen++;
+  }
+  if (nbatches > c->batches_cap) {
+    if (c->d_chunks) hipFree(c->d_chunks);
+    if (c->d_nchunks) hipFree(c->d_nchunks);
+    c->d_chunks = nullptr;
+    c->d_nchunks = nullptr;
+    size_t cap = std::max<size_t>(nbatches, 256);
+    void *p = nullptr, *q = nullptr;
+    if (hipMalloc(&p, cap * kMaxChunks * sizeof(DevChunk)) != hipSuccess) return fail(c, XNH_E_ALLOC, "chunk table");
+    c->d_chunks = static_cast<DevChunk*>(p);
+    if (hipMalloc(&q, cap * sizeof(int32_t)) != hipSuccess) return fail(c, XNH_E_ALLOC, "chunk counts");
+    c->d_nchunks = static_cast<int32_t*>(q);
+    c->batches_cap = cap;
+    c->table_gen++;
+  }
+  c->B.jobs = c->d_jobs;
+  c->B.chunks = c->d_chunks;
+  c->B.nchunks = c->d_nchunks;
+  return XNH_OK;
+}
+
+// Enqueues one batch: the whole bounce loop for `depth` bounces.
+int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
+  hipStream_t s = c->stream;
+  const int cap = c->B.capacity;
+  HIP_OR_FAIL(c, launch_raygen(s, c->B, cap));
+  for (int d = 0; d < depth; ++d) {
+    const int qi = d & 1;
+    if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
+    HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth));
+    if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
+    if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
+    HIP_OR_FAIL(c, launch_shade(s, c->B, qi, d, c->grid_shade));
+    if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
+    if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
+    HIP_OR_FAIL(c, launch_shadow(s, c->B, 2 * d + 1, c->grid_trace, c->block, c->lds_depth));
+    if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
+  }
+  HIP_OR_FAIL(c, launch_accumulate(s, c->B, cap));
+  HIP_OR_FAIL(c, launch_advance(s, c->B.counters));
+  return XNH_OK;
+}
+
+int ensure_graph(xnh_ctx* c, int depth) {
+  if (c->graph && c->graph_depth == depth && c->graph_gen == c->table_gen) return XNH_OK;
+  if (c->graph) {
+    hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  hipGraph_t g = nullptr;
+  HIP_OR_FAIL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_batch(c, depth, nullptr);
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc != XNH_OK) {
+    if (g) hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  const hipError_t e2 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e2 != hipSuccess) {
+    c->graph = nullptr;
+    return fail(c, XNH_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+  }
+  c->graph_depth = depth;
+  c->graph_gen = c->table_gen;
+  return XNH_OK;
+}
+
+struct JobSpec {
+  const DevScene* S;
+  xnh_camera cam;
+  int W, H, spp_first, spp_count, maxb, minb;
+  float4* accum;
+};
+
+// Renders a list of jobs; sums are added to each job's accumulator.
+int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
+  const size_t cap = static_cast<size_t>(c->B.capacity);
+  // batch schedule: greedy, one chunk per job per batch
+  std::vector<DevChunk> chunks;
+  std::vector<int32_t> counts;
+  size_t used = 0;
+  int in_batch = 0;
+  auto new_batch = [&]() {
+    counts.push_back(0);
+    chunks.resize(counts.size() * kMaxChunks);
+    used = 0;
+    in_batch = 0;
+  };
+  int depth = 0;
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    const JobSpec& js = jobs[j];
+    const size_t npix = static_cast<size_t>(js.W) * js.H;
+    if (npix > cap) return fail(c, XNH_E_INVALID, "job larger than the path capacity");
+    depth = std::max(depth, js.maxb);
+    int remaining = js.spp_count, next = js.spp_first;
+    while (remaining > 0) {
+      if (counts.empty() || in_batch == kMaxChunks || used + npix > cap) new_batch();
+      const int ns = static_cast<int>(std::min<size_t>(remaining, (cap - used) / npix));
+      DevChunk& ck = chunks[(counts.size() - 1) * kMaxChunks + in_batch];
+      ck = DevChunk{};
+      ck.job = static_cast<int32_t>(j);
+      ck.spp_first = next;
+      ck.nsamples = ns;
+      ck.npix = static_cast<int32_t>(npix);
+      ck.base = static_cast<int32_t>(used);
+      ck.cap = static_cast<int32_t>(npix * ns);
+      counts.back() = ++in_batch;
+      used += npix * ns;
+      remaining -= ns;
+      next += ns;
+    }
+  }
+  const size_t nb = counts.size();
+  int rc = ensure_tables(c, std::max<size_t>(jobs.size(), 1), std::max<size_t>(nb, 1));
+  if (rc != XNH_OK) return rc;
+  std::vector<DevJob> dj(jobs.size());
+  for (size_t j = 0; j < jobs.size(); ++j) {
+    DevJob& d = dj[j];
+    d = DevJob{};
+    d.S = *jobs[j].S;
+    for (int k = 0; k < 3; ++k) {
+      d.cam.eye[k] = jobs[j].cam.eye[k];
+      d.cam.llc[k] = jobs[j].cam.lower_left[k];
+      d.cam.horiz[k] = jobs[j].cam.horizontal[k];
+      d.cam.vert[k] = jobs[j].cam.vertical[k];
+    }
+    d.W = jobs[j].W;
+    d.H = jobs[j].H;
+    d.minb = jobs[j].minb;
+    d.maxb = jobs[j].maxb;
+    d.accum = jobs[j].accum;
   }
   hipStream_t s = c->stream;
+  HIP_OR_FAIL(c, hipStreamSynchronize(s));
+  if (!dj.empty()) HIP_OR_FAIL(c, hipMemcpy(c->d_jobs, dj.data(), dj.size() * sizeof(DevJob), hipMemcpyHostToDevice));
+  if (nb) {
+    HIP_OR_FAIL(c, hipMemcpy(c->d_chunks, chunks.data(), chunks.size() * sizeof(DevChunk), hipMemcpyHostToDevice));
+    HIP_OR_FAIL(c, hipMemcpy(c->d_nchunks, counts.data(), counts.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
   HIP_OR_FAIL(c, hipMemsetAsync(c->B.stats, 0, sizeof(unsigned long long) * kNumStat, s));
-  HIP_OR_FAIL(c, hipMemsetAsync(c->B.counters + kOverflow, 0, sizeof(uint32_t), s));
-  // per-kernel timing: an event before and after each traversal/shade launch
-  const int nbatches = spp_count == 0 ? 0 : (spp_count + c->nsamples - 1) / c->nsamples;
-  const size_t nev = c->profiling ? static_cast<size_t>(nbatches) * maxb * 6 : 0;
+  HIP_OR_FAIL(c, hipMemsetAsync(c->B.counters, 0, sizeof(uint32_t) * 16, s));   // batch index, overflow
+  const size_t nev = c->profiling ? nb * depth * 6 : 0;
   while (c->prof_events.size() < nev) {
     hipEvent_t e;
     HIP_OR_FAIL(c, hipEventCreate(&e));
     c->prof_events.push_back(e);
   }
+  const bool graph = c->use_graph && !c->profiling && nb > 0;
+  if (graph) {
+    rc = ensure_graph(c, depth);
+    if (rc != XNH_OK) return rc;
+  }
   HIP_OR_FAIL(c, hipEventRecord(c->ev_begin, s));
   size_t ev = 0;
-  int launches = 0;
-  for (int done = 0; done < spp_count;) {
-    const int ns = std::min(c->nsamples, spp_count - done);
-    HIP_OR_FAIL(c, launch_raygen(s, c->B, dc, W, H, spp_first + done, ns, 2 * std::max(maxb, 1)));
-    for (int d = 0; d < maxb; ++d) {
-      const int qi = d & 1;
-      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
-      HIP_OR_FAIL(c, launch_closest(s, c->S, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth));
-      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
-      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
-      HIP_OR_FAIL(c, launch_shade(s, c->S, c->B, qi, d, minb, maxb, c->grid_shade));
-      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
-      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
-      HIP_OR_FAIL(c, launch_shadow(s, c->S, c->B, 2 * d + 1, c->grid_trace, c->block, c->lds_depth));
-      if (c->profiling) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[ev++], s));
-      ++launches;
+  for (size_t b = 0; b < nb; ++b) {
+    if (graph) {
+      HIP_OR_FAIL(c, hipGraphLaunch(c->graph, s));
+    } else {
+      rc = enqueue_batch(c, depth, c->profiling ? &ev : nullptr);
+      if (rc != XNH_OK) return rc;
     }
-    HIP_OR_FAIL(c, launch_accumulate(s, c->B, npix, ns));
-    done += ns;
   }
   HIP_OR_FAIL(c, hipEventRecord(c->ev_end, s));
-  c->spp_total += spp_count;
-  if (rgb_out && c->spp_total > 0) {
-    const float inv = 1.0f / static_cast<float>(c->spp_total);
-    HIP_OR_FAIL(c, launch_average(s, c->B.accum, c->d_out, npix, inv));
-    HIP_OR_FAIL(c, hipMemcpyAsync(rgb_out, c->d_out, sizeof(float) * 3 * npix, hipMemcpyDeviceToHost, s));
-  }
   unsigned long long st[kNumStat];
   uint32_t ovf = 0;
   HIP_OR_FAIL(c, hipMemcpyAsync(st, c->B.stats, sizeof st, hipMemcpyDeviceToHost, s));
@@ -491,7 +563,7 @@ int xnh_render(xnh_ctx* c, const xnh_camera* cam, int32_t spp_first, int32_t spp
   HIP_OR_FAIL(c, hipStreamSynchronize(s));
   xnh_stats& o = c->stats;
   o = xnh_stats{};
-  o.paths = static_cast<uint64_t>(npix) * spp_count;
+  o.paths = st[kStPaths];
   o.closest_rays = st[kStClosest];
   o.shadow_rays = st[kStShadow];
   o.node_tests = st[kStNodeTests];
@@ -506,7 +578,7 @@ int xnh_render(xnh_ctx* c, const xnh_camera* cam, int32_t spp_first, int32_t spp
   float ms = 0.f;
   HIP_OR_FAIL(c, hipEventElapsedTime(&ms, c->ev_begin, c->ev_end));
   o.render_ms = ms;
-  o.closest_launches = launches;
+  o.closest_launches = static_cast<int32_t>(nb * depth);
   o.stack_overflows = static_cast<int32_t>(ovf);
   if (c->profiling) {
     for (size_t k = 0; k + 5 < ev; k += 6) {
@@ -523,12 +595,197 @@ int xnh_render(xnh_ctx* c, const xnh_camera* cam, int32_t spp_first, int32_t spp
   return XNH_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
+int xnh_create(const xnh_config* cfg, xnh_ctx** out) {
+  if (!cfg || !out) return XNH_E_INVALID;
+  *out = nullptr;
+  if (cfg->width <= 0 || cfg->height <= 0 || cfg->max_bounces < 0 || cfg->min_bounces < 0) return XNH_E_INVALID;
+  if (cfg->max_bounces > 64) return XNH_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return XNH_E_NO_DEVICE;
+  xnh_ctx* c = new (std::nothrow) xnh_ctx();
+  if (!c) return XNH_E_ALLOC;
+  c->cfg = *cfg;
+  c->block = cfg->block_size > 0 ? cfg->block_size : 256;
+  if (c->block % 64 != 0 || c->block > 256) {
+    delete c;
+    return XNH_E_INVALID;
+  }
+  c->lds_depth = cfg->lds_stack > 0 ? std::min(cfg->lds_stack, kMaxStack) : 32;
+  c->use_graph = (cfg->flags & XNH_FLAG_NO_GRAPH) == 0;
+  const long long npix = static_cast<long long>(cfg->width) * cfg->height;
+  if (cfg->spp_per_launch > 0) {
+    c->nsamples = cfg->spp_per_launch;
+  } else {
+    const long long target = 8LL << 20;   // paths in flight per batch
+    c->nsamples = static_cast<int>(std::max(1LL, std::min(64LL, target / npix)));
+  }
+  if (npix * c->nsamples > (1LL << 30)) {
+    delete c;
+    return XNH_E_INVALID;
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
+    xnh_destroy(c);
+    return XNH_E_HIP;
+  }
+  c->grid_trace = trace_grid(c->block, c->lds_depth);
+  const int cus = num_cus();
+  if (c->grid_trace <= 0 || cus <= 0) {
+    xnh_destroy(c);
+    return XNH_E_HIP;
+  }
+  c->grid_shade = cus * 8;
+  const int rc = alloc_waves(c);
+  if (rc != XNH_OK) {
+    xnh_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return XNH_OK;
+}
+
+void xnh_destroy(xnh_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->graph) hipGraphExecDestroy(c->graph);
+  free_all(c->own.allocs);
+  free_all(c->wave_allocs);
+  if (c->d_jobs) hipFree(c->d_jobs);
+  if (c->d_chunks) hipFree(c->d_chunks);
+  if (c->d_nchunks) hipFree(c->d_nchunks);
+  for (hipEvent_t e : c->prof_events) hipEventDestroy(e);
+  if (c->ev_begin) hipEventDestroy(c->ev_begin);
+  if (c->ev_end) hipEventDestroy(c->ev_end);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* xnh_last_error(const xnh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int xnh_upload_scene(xnh_ctx* c, const xnh_flat_scene* fs) {
+  if (!c || !fs) return XNH_E_INVALID;
+  c->has_scene = false;
+  const int rc = flatten_upload(c, fs, &c->own);
+  if (rc == XNH_OK) c->has_scene = true;
+  return rc;
+}
+
+int xnh_scene_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene** out) {
+  if (!c || !fs || !out) return XNH_E_INVALID;
+  *out = nullptr;
+  xnh_gpu_scene* g = new (std::nothrow) xnh_gpu_scene();
+  if (!g) return XNH_E_ALLOC;
+  const int rc = flatten_upload(c, fs, g);
+  if (rc != XNH_OK) {
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return XNH_OK;
+}
+
+void xnh_scene_release(xnh_ctx* c, xnh_gpu_scene* g) {
+  if (!g) return;
+  if (c && c->stream) hipStreamSynchronize(c->stream);
+  free_all(g->allocs);
+  delete g;
+}
+
+int xnh_set_profiling(xnh_ctx* c, int32_t on) {
+  if (!c) return XNH_E_INVALID;
+  c->profiling = on != 0;
+  return XNH_OK;
+}
+
+int xnh_render(xnh_ctx* c, const xnh_camera* cam, int32_t spp_first, int32_t spp_count, float* rgb_out) {
+  if (!c || !cam || spp_count < 0 || spp_first < 0) return XNH_E_INVALID;
+  if (!c->has_scene) return fail(c, XNH_E_NO_SCENE, "xnh_render before xnh_upload_scene");
+  const int npix = c->cfg.width * c->cfg.height;
+  std::vector<JobSpec> jobs;
+  if (spp_count > 0)
+    jobs.push_back(JobSpec{&c->own.S, *cam, c->cfg.width, c->cfg.height, spp_first, spp_count, c->cfg.max_bounces,
+                           c->cfg.min_bounces, c->accum});
+  const int rc = render_jobs(c, jobs);
+  if (rc != XNH_OK) return rc;
+  c->spp_total += spp_count;
+  if (rgb_out && c->spp_total > 0) {
+    const float inv = 1.0f / static_cast<float>(c->spp_total);
+    HIP_OR_FAIL(c, launch_average(c->stream, c->accum, c->d_out, npix, inv));
+    HIP_OR_FAIL(c, hipMemcpyAsync(rgb_out, c->d_out, sizeof(float) * 3 * npix, hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  }
+  return XNH_OK;
+}
+
+int xnh_render_jobs(xnh_ctx* c, const xnh_job* jobs, int32_t njobs) {
+  if (!c || njobs < 0 || (njobs > 0 && !jobs)) return XNH_E_INVALID;
+  std::vector<JobSpec> spec;
+  std::vector<void*> tmp;
+  std::vector<float4*> acc(njobs, nullptr);
+  for (int j = 0; j < njobs; ++j) {
+    const xnh_job& jb = jobs[j];
+    if (jb.width <= 0 || jb.height <= 0 || jb.spp_first < 0 || jb.spp_count < 0 || jb.max_bounces < 0 ||
+        jb.max_bounces > 64 || jb.min_bounces < 0) {
+      free_all(tmp);
+      return fail(c, XNH_E_INVALID, "bad job");
+    }
+    const DevScene* S = jb.scene ? &jb.scene->S : (c->has_scene ? &c->own.S : nullptr);
+    if (!S) {
+      free_all(tmp);
+      return fail(c, XNH_E_NO_SCENE, "job without a scene");
+    }
+    const size_t npix = static_cast<size_t>(jb.width) * jb.height;
+    int rc = dev_alloc(c, tmp, &acc[j], npix);
+    if (rc != XNH_OK) {
+      free_all(tmp);
+      return rc;
+    }
+    if (hipMemsetAsync(acc[j], 0, sizeof(float4) * npix, c->stream) != hipSuccess) {
+      free_all(tmp);
+      return fail(c, XNH_E_HIP, "hipMemsetAsync");
+    }
+    spec.push_back(JobSpec{S, jb.camera, jb.width, jb.height, jb.spp_first, jb.spp_count, jb.max_bounces,
+                           jb.min_bounces, acc[j]});
+  }
+  int rc = render_jobs(c, spec);
+  for (int j = 0; j < njobs && rc == XNH_OK; ++j) {
+    const xnh_job& jb = jobs[j];
+    const size_t npix = static_cast<size_t>(jb.width) * jb.height;
+    if (!jb.rgb_out && !jb.sum_out) continue;
+    std::vector<float4> h(npix);
+    if (hipMemcpy(h.data(), acc[j], sizeof(float4) * npix, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = fail(c, XNH_E_HIP, "hipMemcpy");
+      break;
+    }
+    const float inv = jb.spp_count > 0 ? 1.0f / static_cast<float>(jb.spp_count) : 0.f;
+    for (size_t p = 0; p < npix; ++p) {
+      if (jb.sum_out) {
+        jb.sum_out[3 * p] = h[p].x;
+        jb.sum_out[3 * p + 1] = h[p].y;
+        jb.sum_out[3 * p + 2] = h[p].z;
+      }
+      if (jb.rgb_out) {
+        jb.rgb_out[3 * p] = h[p].x * inv;
+        jb.rgb_out[3 * p + 1] = h[p].y * inv;
+        jb.rgb_out[3 * p + 2] = h[p].z * inv;
+      }
+    }
+  }
+  hipStreamSynchronize(c->stream);
+  free_all(tmp);
+  return rc;
+}
+
 int xnh_read_accum(xnh_ctx* c, float* sum_out, int32_t* spp_out) {
   if (!c) return XNH_E_INVALID;
   const int npix = c->cfg.width * c->cfg.height;
   if (sum_out) {
     std::vector<float4> tmp(npix);
-    HIP_OR_FAIL(c, hipMemcpyAsync(tmp.data(), c->B.accum, sizeof(float4) * npix, hipMemcpyDeviceToHost, c->stream));
+    HIP_OR_FAIL(c, hipMemcpyAsync(tmp.data(), c->accum, sizeof(float4) * npix, hipMemcpyDeviceToHost, c->stream));
     HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
     for (int p = 0; p < npix; ++p) {
       sum_out[3 * p] = tmp[p].x;
@@ -542,7 +799,7 @@ int xnh_read_accum(xnh_ctx* c, float* sum_out, int32_t* spp_out) {
 
 int xnh_reset(xnh_ctx* c) {
   if (!c) return XNH_E_INVALID;
-  HIP_OR_FAIL(c, hipMemsetAsync(c->B.accum, 0, sizeof(float4) * c->cfg.width * c->cfg.height, c->stream));
+  HIP_OR_FAIL(c, hipMemsetAsync(c->accum, 0, sizeof(float4) * c->cfg.width * c->cfg.height, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   c->spp_total = 0;
   return XNH_OK;
@@ -590,9 +847,9 @@ static int trace_common(xnh_ctx* c, const xnh_ray* rays, int32_t n, bool any, vo
   if (e == hipSuccess) e = hipMemsetAsync(d_ovf, 0, sizeof(uint32_t), c->stream);
   if (e == hipSuccess) e = hipEventRecord(c->ev_begin, c->stream);
   if (e == hipSuccess) {
-    e = any ? launch_api_any(c->stream, c->S, d_rays, static_cast<uint8_t*>(d_out), n, d_spill, c->lds_depth, d_stats,
+    e = any ? launch_api_any(c->stream, c->own.S, d_rays, static_cast<uint8_t*>(d_out), n, d_spill, c->lds_depth, d_stats,
                              d_ovf)
-            : launch_api_closest(c->stream, c->S, d_rays, static_cast<float*>(d_out), n, d_spill, c->lds_depth,
+            : launch_api_closest(c->stream, c->own.S, d_rays, static_cast<float*>(d_out), n, d_spill, c->lds_depth,
                                  d_stats, d_ovf);
   }
   if (e == hipSuccess) e = hipEventRecord(c->ev_end, c->stream);
@@ -629,168 +886,6 @@ int xnh_trace_closest(xnh_ctx* c, const xnh_ray* rays, int32_t n, xnh_hit* hits)
 }
 int xnh_trace_any(xnh_ctx* c, const xnh_ray* rays, int32_t n, uint8_t* occ) {
   return trace_common(c, rays, n, true, occ);
-}
-
-// ---- host scene builder ----------------------------------------------------
-
-xnh_scene* xnh_scene_new(void) { return new (std::nothrow) xnh_scene(); }
-void xnh_scene_free(xnh_scene* s) { delete s; }
-
-int xnh_scene_add_material(xnh_scene* s, const float* p) {
-  if (!s || !p) return XNH_E_INVALID;
-  s->s.materials.push_back(prepare_material(p));
-  s->s.built = false;
-  return static_cast<int>(s->s.materials.size()) - 1;
-}
-
-int xnh_scene_add_mesh(xnh_scene* s, int32_t nv, const float* pos, const float* nrm, int32_t nt,
-                       const int32_t* indices, const int32_t* mat_ids) {
-  if (!s || nv < 0 || nt < 0 || (nv > 0 && !pos) || (nt > 0 && (!indices || !mat_ids))) return XNH_E_INVALID;
-  HostMesh m;
-  m.vx.resize(nv);
-  m.vy.resize(nv);
-  m.vz.resize(nv);
-  for (int i = 0; i < nv; ++i) {
-    m.vx[i] = pos[3 * i];
-    m.vy[i] = pos[3 * i + 1];
-    m.vz[i] = pos[3 * i + 2];
-  }
-  m.indices.assign(indices, indices + 3 * static_cast<size_t>(nt));
-  m.mat_ids.assign(mat_ids, mat_ids + nt);
-  for (int32_t v : m.indices)
-    if (v < 0 || v >= nv) return XNH_E_INVALID;
-  if (nrm) {
-    m.nx.resize(nv);
-    m.ny.resize(nv);
-    m.nz.resize(nv);
-    for (int i = 0; i < nv; ++i) {
-      m.nx[i] = nrm[3 * i];
-      m.ny[i] = nrm[3 * i + 1];
-      m.nz[i] = nrm[3 * i + 2];
-    }
-  } else {
-    smooth_normals(m);
-  }
-  s->s.meshes.push_back(std::move(m));
-  s->s.built = false;
-  return static_cast<int>(s->s.meshes.size()) - 1;
-}
-
-int xnh_scene_add_light(xnh_scene* s, int32_t mesh, int32_t tri, const float* e, float area) {
-  if (!s || !e || mesh < 0 || tri < 0) return XNH_E_INVALID;
-  xnh_light l{};
-  l.mesh_id = static_cast<uint32_t>(mesh);
-  l.tri_idx = static_cast<uint32_t>(tri);
-  l.emission[0] = e[0];
-  l.emission[1] = e[1];
-  l.emission[2] = e[2];
-  l.area = area;
-  s->s.lights.push_back(l);
-  return static_cast<int>(s->s.lights.size()) - 1;
-}
-
-// Normalised as the scene parser does (scene_file.cpp:137-140).
-int xnh_scene_add_dir_light(xnh_scene* s, const float* d, const float* color, float intensity) {
-  if (!s || !d || !color) return XNH_E_INVALID;
-  xnh_dir_light l{};
-  const float len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-  if (len > 1e-6f) {
-    const float r = 1.f / len;
-    l.direction[0] = d[0] * r;
-    l.direction[1] = d[1] * r;
-    l.direction[2] = d[2] * r;
-  } else {
-    l.direction[0] = 0.f;
-    l.direction[1] = 1.f;
-    l.direction[2] = 0.f;
-  }
-  for (int k = 0; k < 3; ++k) l.emission[k] = color[k] * intensity;
-  s->s.suns.push_back(l);
-  return static_cast<int>(s->s.suns.size()) - 1;
-}
-
-int xnh_scene_set_sky(xnh_scene* s, const float* z, const float* h, const float* g, float intensity, float sharp) {
-  if (!s || !z || !h || !g) return XNH_E_INVALID;
-  for (int k = 0; k < 3; ++k) {
-    s->s.sky.zenith[k] = z[k];
-    s->s.sky.horizon[k] = h[k];
-    s->s.sky.ground[k] = g[k];
-  }
-  s->s.sky.intensity = intensity;
-  s->s.sky.sharpness = sharp;
-  return XNH_OK;
-}
-
-int xnh_scene_add_instance(xnh_scene* s, int32_t mesh, const float* t, float scale, const float* rot, int32_t id) {
-  if (!s || !t || !rot || mesh < 0) return XNH_E_INVALID;
-  PendingInstance p{};
-  p.mesh = mesh;
-  p.id = id;
-  p.s = scale;
-  for (int k = 0; k < 3; ++k) {
-    p.t[k] = t[k];
-    p.rot[k] = rot[k];
-  }
-  s->s.pending.push_back(p);
-  s->s.built = false;
-  return static_cast<int>(s->s.pending.size()) - 1;
-}
-
-int xnh_scene_build(xnh_scene* s, int32_t nthreads) {
-  (void)nthreads;
-  if (!s) return XNH_E_INVALID;
-  HostScene& hs = s->s;
-  for (const auto& p : hs.pending)
-    if (p.mesh >= static_cast<int32_t>(hs.meshes.size())) return XNH_E_INVALID;
-  for (auto& m : hs.meshes) build_blas(m);
-  build_tlas(hs);
-  hs.mesh_views.resize(hs.meshes.size());
-  for (size_t i = 0; i < hs.meshes.size(); ++i) {
-    HostMesh& m = hs.meshes[i];
-    xnh_mesh& v = hs.mesh_views[i];
-    v.num_vertices = static_cast<int32_t>(m.vx.size());
-    v.num_triangles = m.tris();
-    v.vx = m.vx.data();
-    v.vy = m.vy.data();
-    v.vz = m.vz.data();
-    v.nx = m.nx.empty() ? nullptr : m.nx.data();
-    v.ny = m.ny.empty() ? nullptr : m.ny.data();
-    v.nz = m.nz.empty() ? nullptr : m.nz.data();
-    v.indices = m.indices.data();
-    v.mat_ids = m.mat_ids.data();
-    v.num_nodes = static_cast<int32_t>(m.nodes.size());
-    v.nodes = m.nodes.data();
-    v.prims = m.prims.data();
-  }
-  hs.built = true;
-  return XNH_OK;
-}
-
-int xnh_scene_flat(const xnh_scene* s, xnh_flat_scene* out) {
-  if (!s || !out || !s->s.built) return XNH_E_INVALID;
-  const HostScene& hs = s->s;
-  out->num_meshes = static_cast<int32_t>(hs.mesh_views.size());
-  out->meshes = hs.mesh_views.data();
-  out->num_instances = static_cast<int32_t>(hs.instances.size());
-  out->instances = hs.instances.data();
-  out->num_tlas_nodes = static_cast<int32_t>(hs.tlas_nodes.size());
-  out->tlas_nodes = hs.tlas_nodes.data();
-  out->tlas_inst_indices = hs.tlas_order.data();
-  out->num_materials = static_cast<int32_t>(hs.materials.size());
-  out->materials = hs.materials.data();
-  out->num_lights = static_cast<int32_t>(hs.lights.size());
-  out->lights = hs.lights.data();
-  out->num_dir_lights = static_cast<int32_t>(hs.suns.size());
-  out->dir_lights = hs.suns.data();
-  out->sky = hs.sky;
-  return XNH_OK;
-}
-
-int xnh_camera_look_at(const float* eye, const float* target, const float* up, float fov, float aspect,
-                       xnh_camera* out) {
-  if (!eye || !target || !up || !out) return XNH_E_INVALID;
-  *out = look_at(eye, target, up, fov, aspect);
-  return XNH_OK;
 }
 
 }  // extern "C"

@@ -93,6 +93,12 @@ class Hit(C.Structure):
                 ("geo_normal", C.c_float * 3), ("front_face", C.c_int32)]
 
 
+class Job(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("width", C.c_int32), ("height", C.c_int32),
+                ("spp_first", C.c_int32), ("spp_count", C.c_int32), ("max_bounces", C.c_int32),
+                ("min_bounces", C.c_int32), ("rgb_out", _fp), ("sum_out", _fp)]
+
+
 class Stats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("closest_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("node_tests", C.c_uint64), ("interior", C.c_uint64), ("tri_tests", C.c_uint64),
@@ -119,6 +125,9 @@ SIGNATURES = {
     "xnh_set_profiling": (C.c_int, [_vp, C.c_int32]),
     "xnh_trace_closest": (C.c_int, [_vp, C.POINTER(Ray), C.c_int32, C.POINTER(Hit)]),
     "xnh_trace_any": (C.c_int, [_vp, C.POINTER(Ray), C.c_int32, C.POINTER(C.c_uint8)]),
+    "xnh_scene_upload": (C.c_int, [_vp, C.POINTER(FlatScene), C.POINTER(_vp)]),
+    "xnh_scene_release": (None, [_vp, _vp]),
+    "xnh_render_jobs": (C.c_int, [_vp, C.POINTER(Job), C.c_int32]),
     "xnh_scene_new": (_vp, []),
     "xnh_scene_free": (None, [_vp]),
     "xnh_scene_add_material": (C.c_int, [_vp, _fp]),

@@ -121,15 +121,31 @@ class HipScene:
         return nodes, order, xf, ids, meshes
 
 
+class GpuScene:
+    """A scene kept resident on the device for batched jobs (xnh_scene_upload)."""
+
+    def __init__(self, renderer: "HipRenderer", scene: HipScene):
+        self.renderer = renderer
+        self.scene = scene
+        h = C.c_void_p()
+        renderer._chk(N.lib().xnh_scene_upload(renderer.h, C.byref(scene.flat), C.byref(h)))
+        self.h = h
+
+    def release(self):
+        if getattr(self, "h", None) and self.renderer.h:
+            N.lib().xnh_scene_release(self.renderer.h, self.h)
+        self.h = None
+
+
 class HipRenderer:
     """The backend renderer: ``CudaRenderer``'s three-method contract plus
     batch rendering and the ray-batch parity API."""
 
     def __init__(self, width: int, height: int, max_bounces: int = 8, min_bounces: int = 3,
-                 spp_per_launch: int = 0, block_size: int = 0, lds_stack: int = 0):
+                 spp_per_launch: int = 0, block_size: int = 0, lds_stack: int = 0, use_graph: bool = True):
         self.width, self.height = int(width), int(height)
         cfg = N.Config(self.width, self.height, int(max_bounces), int(min_bounces), int(spp_per_launch),
-                       int(block_size), int(lds_stack), 0)
+                       int(block_size), int(lds_stack), 0 if use_graph else 1)
         h = C.c_void_p()
         N.check(N.lib().xnh_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -192,6 +208,26 @@ class HipRenderer:
         s = N.Stats()
         self._chk(N.lib().xnh_get_stats(self.h, C.byref(s)))
         return s.as_dict()
+
+    def render_jobs(self, jobs, want_sums: bool = False):
+        """Renders a list of jobs together. Each job is a dict with keys
+        scene (GpuScene or None for the uploaded scene), camera, width,
+        height, spp_first, spp_count, max_bounces, min_bounces. Returns one
+        (H*W, 3) average per job (and the sums when want_sums)."""
+        arr = (N.Job * len(jobs))()
+        outs, sums = [], []
+        for k, j in enumerate(jobs):
+            W, H = int(j["width"]), int(j["height"])
+            o = np.empty((W * H, 3), np.float32)
+            sm = np.empty((W * H, 3), np.float32) if want_sums else None
+            outs.append(o)
+            sums.append(sm)
+            sc = j.get("scene")
+            arr[k] = N.Job(sc.h if sc is not None else None, j["camera"], W, H, int(j.get("spp_first", 1)),
+                           int(j["spp_count"]), int(j["max_bounces"]), int(j["min_bounces"]), _fp(o),
+                           _fp(sm) if sm is not None else None)
+        self._chk(N.lib().xnh_render_jobs(self.h, arr, len(jobs)))
+        return (outs, sums) if want_sums else outs
 
     # -- ray batches ----------------------------------------------------------
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
