@@ -218,6 +218,37 @@ void xnh_scene_release(xnh_ctx* ctx, xnh_gpu_scene* scene);
  * the reference. */
 int xnh_render_jobs(xnh_ctx* ctx, const xnh_job* jobs, int32_t njobs);
 
+/* ---- path guiding (no counterpart in the reference) ---------------------
+ * A multiresolution hash grid (16 levels x 2^t_log2 entries x 2 FP16
+ * features) and a bias-free 3x64 ReLU MLP map (position, normal) to an
+ * 8-lobe von Mises-Fisher mixture over incident direction; FP16 MFMA
+ * forward/backward with FP32 accumulation, Adam on FP32 master weights.
+ * Raw network outputs are 48 floats per point: lobe j uses
+ * [5j .. 5j+4] = mean (unnormalised xyz), log kappa, weight logit. */
+typedef struct {
+  int32_t t_log2;          /* table entries per level = 2^t_log2 (0: 19) */
+  float learning_rate;     /* Adam step size (0: 5e-3) */
+  float guide_fraction;    /* one-sample MIS: probability of sampling the guide (0: 0.5) */
+  uint32_t seed;           /* parameter initialisation */
+  float bounds_lo[3], bounds_hi[3];   /* scene box mapped to the unit cube */
+  int32_t train_batch;     /* training samples per step (0: 2^18) */
+  int32_t record_paths;    /* paths per batch that record training samples (0: 2^16) */
+} xnh_guide_config;
+
+int xnh_guide_enable(xnh_ctx* ctx, const xnh_guide_config* cfg);
+int xnh_guide_disable(xnh_ctx* ctx);
+/* FP32 master parameters: grid [16][2^t_log2][2], weights 48x64, 64x64,
+ * 64x64, 64x48 (row-major [in][out], concatenated). */
+int xnh_guide_set_params(xnh_ctx* ctx, const float* grid, const float* weights);
+int xnh_guide_get_params(xnh_ctx* ctx, float* grid, float* weights);
+int xnh_guide_eval(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm, float* raw_out);
+/* One optimisation step on a host batch: loss -lambda_i log q(dir_i) with
+ * lambda scaled to mean 1. With apply == 0 only gradients are computed and,
+ * when grid_grad / w_grad are non-null, returned (sums over the batch of the
+ * scaled loss); with apply != 0 Adam updates the parameters. */
+int xnh_guide_train(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm, const float* dir, const float* lam,
+                    int32_t apply, float* loss_out, float* grid_grad, float* w_grad);
+
 /* Copies the raw per-pixel sums (W*H*3) and returns the sample count. */
 int xnh_read_accum(xnh_ctx* ctx, float* sum_out, int32_t* spp_out);
 int xnh_reset(xnh_ctx* ctx);

@@ -1,0 +1,210 @@
+// guide.h — the path-guiding network on the device: a multiresolution hash
+// grid and a bias-free 3x64 ReLU MLP mapping (position, normal) to an
+// 8-lobe von Mises-Fisher mixture over incident direction.
+//
+// Methods: hash encoding of Mueller et al. 2022 (levels, geometric
+// resolutions, spatial hash x*1 ^ y*2654435761 ^ z*805459861, dense tables
+// where a level fits, trilinear weights); real spherical harmonics of degree
+// <= 3 for the normal; vMF lobes with C(k) = k / (2 pi (1 - e^{-2k})) and
+// Jakob's 2012 stable inverse-CDF sampler; softmax lobe weights. No
+// counterpart in the reference renderer.
+//
+// Layout: grid params FP16 [level][slot][2]; MLP weights FP16 row-major
+// [in][out] per layer (48x64, 64x64, 64x64, 64x48). Matrix products use
+// v_mfma_f32_16x16x16f16 (FP16 operands, FP32 accumulation), one wave per
+// 64 samples; operand lane layout (checked on gfx950): lane l holds
+// A[l%16][4(l/16)+j], B[4(l/16)+j][l%16] and D[4(l/16)+j][l%16], j = 0..3.
+// Functions marked "wave" must be reached by all 64 lanes of a one-wave
+// block (they use __syncthreads as the LDS barrier).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../common/guide_net.h"
+#include "dmath.h"
+
+namespace xnh {
+
+typedef _Float16 h4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+constexpr float kKappaLogMin = -4.0f, kKappaLogMax = 8.0f;
+constexpr float kPosMax = 0.99999994f;   // largest float below 1
+constexpr float kGuidePi = 3.14159265358979323846f;
+
+struct Mixture {
+  f3 mu[kGLobes];
+  float kappa[kGLobes];
+  float pi[kGLobes];
+};
+
+// ---- encoding ---------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t grid_slot(int ix, int iy, int iz, int res, int t_log2) {
+  const uint32_t n = static_cast<uint32_t>(res) + 1u;
+  const uint64_t n3 = static_cast<uint64_t>(n) * n * n;
+  if (n3 <= (1ull << t_log2)) return static_cast<uint32_t>(ix) + static_cast<uint32_t>(iy) * n +
+                                     static_cast<uint32_t>(iz) * n * n;
+  const uint32_t h = (static_cast<uint32_t>(ix) * 1u) ^ (static_cast<uint32_t>(iy) * 2654435761u) ^
+                     (static_cast<uint32_t>(iz) * 805459861u);
+  return h & ((1u << t_log2) - 1u);
+}
+
+__device__ __forceinline__ f3 guide_unit_pos(const GuideNet& net, f3 p) {
+  f3 q{(p.x - net.lo[0]) * net.inv_ext[0], (p.y - net.lo[1]) * net.inv_ext[1], (p.z - net.lo[2]) * net.inv_ext[2]};
+  q.x = fminf(fmaxf(q.x, 0.f), kPosMax);
+  q.y = fminf(fmaxf(q.y, 0.f), kPosMax);
+  q.z = fminf(fmaxf(q.z, 0.f), kPosMax);
+  return q;
+}
+
+// Trilinear corner of level `l`: slot and weight of corner c (0..7).
+__device__ __forceinline__ void grid_corner(const GuideNet& net, f3 q, int l, int c, uint32_t& slot, float& w) {
+  const float fr = static_cast<float>(net.res[l]);
+  const float x = q.x * fr, y = q.y * fr, z = q.z * fr;
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const float ax = x - fx, ay = y - fy, az = z - fz;
+  const int dx = c & 1, dy = (c >> 1) & 1, dz = (c >> 2) & 1;
+  w = (dx ? ax : 1.f - ax) * (dy ? ay : 1.f - ay) * (dz ? az : 1.f - az);
+  slot = grid_slot(static_cast<int>(fx) + dx, static_cast<int>(fy) + dy, static_cast<int>(fz) + dz, net.res[l],
+                   net.t_log2);
+}
+
+// Real SH, degrees 0..3, of a unit vector.
+__device__ __forceinline__ void sh16(f3 n, float* o) {
+  const float x = n.x, y = n.y, z = n.z;
+  const float xx = x * x, yy = y * y, zz = z * z;
+  o[0] = 0.28209479177387814f;
+  o[1] = -0.48860251190291987f * y;
+  o[2] = 0.48860251190291987f * z;
+  o[3] = -0.48860251190291987f * x;
+  o[4] = 1.0925484305920792f * x * y;
+  o[5] = -1.0925484305920792f * y * z;
+  o[6] = 0.94617469575755997f * zz - 0.31539156525251999f;
+  o[7] = -1.0925484305920792f * x * z;
+  o[8] = 0.54627421529603959f * (xx - yy);
+  o[9] = 0.59004358992664352f * y * (-3.0f * xx + yy);
+  o[10] = 2.8906114426405538f * x * y * z;
+  o[11] = 0.45704579946446572f * y * (1.0f - 5.0f * zz);
+  o[12] = 0.3731763325901154f * z * (5.0f * zz - 3.0f);
+  o[13] = 0.45704579946446572f * x * (1.0f - 5.0f * zz);
+  o[14] = 1.4453057213202769f * z * (xx - yy);
+  o[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
+}
+
+// Writes the 48 FP16 encoding values of (p, n) to `row`.
+__device__ __forceinline__ void guide_encode(const GuideNet& net, f3 p, f3 n, _Float16* row) {
+  const f3 q = guide_unit_pos(net, p);
+  const uint32_t T = 1u << net.t_log2;
+  for (int l = 0; l < kGLevels; ++l) {
+    float a0 = 0.f, a1 = 0.f;
+    const _Float16* g = reinterpret_cast<const _Float16*>(net.grid) + static_cast<size_t>(l) * T * 2u;
+    for (int c = 0; c < 8; ++c) {
+      uint32_t s;
+      float w;
+      grid_corner(net, q, l, c, s, w);
+      const uint32_t pair = *reinterpret_cast<const uint32_t*>(g + 2u * s);
+      _Float16 h0, h1;
+      __builtin_memcpy(&h0, &pair, 2);
+      __builtin_memcpy(&h1, reinterpret_cast<const char*>(&pair) + 2, 2);
+      a0 = a0 + w * static_cast<float>(h0);
+      a1 = a1 + w * static_cast<float>(h1);
+    }
+    row[2 * l] = static_cast<_Float16>(a0);
+    row[2 * l + 1] = static_cast<_Float16>(a1);
+  }
+  float s[16];
+  sh16(n, s);
+  for (int k = 0; k < 16; ++k) row[32 + k] = static_cast<_Float16>(s[k]);
+}
+
+// ---- MFMA tiles -----------------------------------------------------------------
+
+// 16x16 tile of op(A) * op(B) over K (multiple of 16). A: 64 x K rows in LDS
+// (row-major, lda), rows r0..r0+15 (or its transpose when TA: element (m, k)
+// read at A[k * lda + m]); B: K x N row-major (ldb), cols c0..c0+15 (or its
+// transpose when TB: element (k, n) read at B[n * ldb + k]).
+template <bool TA, bool TB>
+__device__ __forceinline__ f4v mma_tile(const _Float16* A, int lda, int r0, const _Float16* B, int ldb, int c0,
+                                        int K) {
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  const int li = lane & 15, lk = 4 * (lane >> 4);
+  f4v acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    h4v a, b;
+    for (int j = 0; j < 4; ++j) {
+      a[j] = TA ? A[(k0 + lk + j) * lda + r0 + li] : A[(r0 + li) * lda + k0 + lk + j];
+      b[j] = TB ? B[(c0 + li) * ldb + k0 + lk + j] : B[(k0 + lk + j) * ldb + c0 + li];
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// wave: Y (64 x N) = act(X (64 x K) W (K x N)); X, Y in LDS, W global.
+// relu: store FP16 ReLU output; else store FP32 (Y32).
+__device__ __forceinline__ void dense_fwd(const _Float16* X, int K, const _Float16* W, int N, _Float16* Y, float* Y32) {
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  for (int rt = 0; rt < 4; ++rt)
+    for (int ct = 0; ct < N / 16; ++ct) {
+      const f4v acc = mma_tile<false, false>(X, K, 16 * rt, W, N, 16 * ct, K);
+      for (int j = 0; j < 4; ++j) {
+        const int r = 16 * rt + 4 * (lane >> 4) + j, col = 16 * ct + (lane & 15);
+        if (Y) Y[r * N + col] = static_cast<_Float16>(fmaxf(acc[j], 0.f));
+        else Y32[r * N + col] = acc[j];
+      }
+    }
+  __syncthreads();
+}
+
+// wave: the full forward pass; lds_x holds the encodings (64 x 48).
+__device__ __forceinline__ void mlp_forward(const GuideNet& net, const _Float16* X, _Float16* H1, _Float16* H2,
+                                            _Float16* H3, float* out) {
+  const _Float16* w = reinterpret_cast<const _Float16*>(net.w);
+  dense_fwd(X, kGIn, w + kW1, kGHid, H1, nullptr);
+  dense_fwd(H1, kGHid, w + kW2, kGHid, H2, nullptr);
+  dense_fwd(H2, kGHid, w + kW3, kGHid, H3, nullptr);
+  dense_fwd(H3, kGHid, w + kW4, kGOut, nullptr, out);
+}
+
+// ---- mixture -----------------------------------------------------------------
+
+__device__ __forceinline__ float vmf_norm(float k) { return k / (2.f * kGuidePi * -expm1f(-2.f * k)); }
+
+__device__ __forceinline__ void decode_mixture(const float* raw, Mixture& m) {
+  float amax = -3.0e38f;
+  for (int j = 0; j < kGLobes; ++j) amax = fmaxf(amax, raw[5 * j + 4]);
+  float den = 0.f;
+  for (int j = 0; j < kGLobes; ++j) {
+    const float* r = raw + 5 * j;
+    const float nm = sqrtf(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + 1e-12f);
+    m.mu[j] = f3{r[0] / nm, r[1] / nm, r[2] / nm};
+    m.kappa[j] = expf(fminf(fmaxf(r[3], kKappaLogMin), kKappaLogMax));
+    m.pi[j] = expf(r[4] - amax);
+    den += m.pi[j];
+  }
+  for (int j = 0; j < kGLobes; ++j) m.pi[j] = m.pi[j] / den;
+}
+
+__device__ __forceinline__ float mixture_pdf(const Mixture& m, f3 w) {
+  float q = 0.f;
+  for (int j = 0; j < kGLobes; ++j)
+    q += m.pi[j] * vmf_norm(m.kappa[j]) * expf(m.kappa[j] * (dot(m.mu[j], w) - 1.f));
+  return q;
+}
+
+__device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u1, float u2) {
+  int j = 0;
+  float cdf = m.pi[0];
+  while (j < kGLobes - 1 && cdf < u0) {
+    ++j;
+    cdf += m.pi[j];
+  }
+  const float k = m.kappa[j];
+  const float wz = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k;
+  const float st = sqrtf(fmaxf(0.f, 1.f - wz * wz));
+  const float phi = 2.f * kGuidePi * u2;
+  const Frame fr(m.mu[j]);
+  return fr.world(f3{st * cosf(phi), st * sinf(phi), wz});
+}
+
+}  // namespace xnh

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../common/dev_types.h"
+#include "../common/guide_net.h"
 
 namespace xnh {
 
@@ -22,5 +23,15 @@ hipError_t launch_api_closest(hipStream_t s, const DevScene& S, const float4* ra
                               uint32_t* spill, int lds_depth, unsigned long long* stats, uint32_t* overflow_ctr);
 hipError_t launch_api_any(hipStream_t s, const DevScene& S, const float4* rays, uint8_t* occ, int n, uint32_t* spill,
                           int lds_depth, unsigned long long* stats, uint32_t* overflow_ctr);
+
+hipError_t launch_guide_eval(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
+                             float* raw);
+hipError_t launch_guide_train(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
+                              const float* dir, const float* lam, float lam_scale, float* grid_g, float* w_g,
+                              float* loss_sum);
+hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, float* grad, float* m, float* v,
+                             size_t count, float lr, float b1, float b2, float eps, float bc1, float bc2,
+                             float grad_scale);
+hipError_t launch_to_half(hipStream_t s, const float* src, uint16_t* dst, size_t count);
 
 }  // namespace xnh

@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "../common/dev_types.h"
+#include "../common/guide_net.h"
 #include "../device/launch.h"
 #include "xenon_hip.h"
 
@@ -40,8 +41,25 @@ struct xnh_gpu_scene {
   size_t bytes = 0;
 };
 
+// Path-guiding network state: FP32 masters, FP16 copies, gradients, Adam
+// moments, and the device training batch.
+struct GuideState {
+  bool on = false;
+  xnh_guide_config cfg{};
+  GuideNet net{};
+  std::vector<void*> allocs;
+  float *grid_m = nullptr, *grid_g = nullptr, *grid_am = nullptr, *grid_av = nullptr;
+  uint16_t* grid_h = nullptr;
+  float *w_m = nullptr, *w_g = nullptr, *w_am = nullptr, *w_av = nullptr;
+  uint16_t* w_h = nullptr;
+  float* loss = nullptr;
+  size_t ngrid = 0;
+  int step = 0;
+};
+
 struct xnh_ctx {
   xnh_config cfg{};
+  GuideState guide;
   std::string err;
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -654,6 +672,7 @@ void xnh_destroy(xnh_ctx* c) {
   if (c->graph) hipGraphExecDestroy(c->graph);
   free_all(c->own.allocs);
   free_all(c->wave_allocs);
+  free_all(c->guide.allocs);
   if (c->d_jobs) hipFree(c->d_jobs);
   if (c->d_chunks) hipFree(c->d_chunks);
   if (c->d_nchunks) hipFree(c->d_nchunks);
@@ -811,6 +830,187 @@ int xnh_get_stats(const xnh_ctx* c, xnh_stats* out) {
   if (!c || !out) return XNH_E_INVALID;
   *out = c->stats;
   return XNH_OK;
+}
+
+// ---- path guiding -------------------------------------------------------------
+
+static uint64_t mix64(uint64_t& s) {   // xorshift64* for parameter initialisation
+  s ^= s >> 12;
+  s ^= s << 25;
+  s ^= s >> 27;
+  return s * 2685821657736338717ULL;
+}
+
+int xnh_guide_disable(xnh_ctx* c) {
+  if (!c) return XNH_E_INVALID;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  free_all(c->guide.allocs);
+  c->guide = GuideState{};
+  return XNH_OK;
+}
+
+int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
+  if (!c || !cfg_in) return XNH_E_INVALID;
+  xnh_guide_config cfg = *cfg_in;
+  if (cfg.t_log2 == 0) cfg.t_log2 = 19;
+  if (cfg.learning_rate <= 0.f) cfg.learning_rate = 5e-3f;
+  if (cfg.guide_fraction <= 0.f) cfg.guide_fraction = 0.5f;
+  if (cfg.train_batch <= 0) cfg.train_batch = 1 << 18;
+  if (cfg.record_paths <= 0) cfg.record_paths = 1 << 16;
+  if (cfg.t_log2 < 8 || cfg.t_log2 > 24 || cfg.guide_fraction >= 1.f) return fail(c, XNH_E_INVALID, "guide config");
+  xnh_guide_disable(c);
+  GuideState& g = c->guide;
+  g.cfg = cfg;
+  const size_t T = size_t(1) << cfg.t_log2;
+  g.ngrid = size_t(kGLevels) * T * kGFeat;
+  int rc = dev_alloc(c, g.allocs, &g.grid_m, g.ngrid);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.grid_g, g.ngrid);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.grid_am, g.ngrid);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.grid_av, g.ngrid);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.grid_h, g.ngrid);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_m, size_t(kWTotal));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_g, size_t(kWTotal));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_am, size_t(kWTotal));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_av, size_t(kWTotal));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_h, size_t(kWTotal));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.loss, size_t(1));
+  if (rc != XNH_OK) {
+    xnh_guide_disable(c);
+    return rc;
+  }
+  g.net.grid = g.grid_h;
+  g.net.w = g.w_h;
+  g.net.t_log2 = cfg.t_log2;
+  const double b = std::exp((std::log(2048.0) - std::log(16.0)) / (kGLevels - 1));
+  for (int l = 0; l < kGLevels; ++l) g.net.res[l] = static_cast<int32_t>(std::floor(16.0 * std::pow(b, l)));
+  for (int k = 0; k < 3; ++k) {
+    g.net.lo[k] = cfg.bounds_lo[k];
+    const float e = cfg.bounds_hi[k] - cfg.bounds_lo[k];
+    g.net.inv_ext[k] = e > 0.f ? 1.f / e : 1.f;
+  }
+  // initial parameters: grid U(-1e-4, 1e-4), weights Glorot-uniform
+  std::vector<float> grid(g.ngrid), w(kWTotal);
+  uint64_t st = 0x9E3779B97F4A7C15ULL ^ (uint64_t(cfg.seed) + 1);
+  auto uni = [&]() { return static_cast<float>((mix64(st) >> 40) * (1.0 / 16777216.0)) * 2.f - 1.f; };
+  for (auto& x : grid) x = 1e-4f * uni();
+  const int dims[4][2] = {{kGIn, kGHid}, {kGHid, kGHid}, {kGHid, kGHid}, {kGHid, kGOut}};
+  const int offs[4] = {kW1, kW2, kW3, kW4};
+  for (int L = 0; L < 4; ++L) {
+    const float a = std::sqrt(6.f / float(dims[L][0] + dims[L][1]));
+    for (int i = 0; i < dims[L][0] * dims[L][1]; ++i) w[offs[L] + i] = a * uni();
+  }
+  g.on = true;
+  return xnh_guide_set_params(c, grid.data(), w.data());
+}
+
+int xnh_guide_set_params(xnh_ctx* c, const float* grid, const float* w) {
+  if (!c || !c->guide.on || !grid || !w) return XNH_E_INVALID;
+  GuideState& g = c->guide;
+  hipStream_t s = c->stream;
+  HIP_OR_FAIL(c, hipMemcpyAsync(g.grid_m, grid, g.ngrid * 4, hipMemcpyHostToDevice, s));
+  HIP_OR_FAIL(c, hipMemcpyAsync(g.w_m, w, size_t(kWTotal) * 4, hipMemcpyHostToDevice, s));
+  HIP_OR_FAIL(c, launch_to_half(s, g.grid_m, g.grid_h, g.ngrid));
+  HIP_OR_FAIL(c, launch_to_half(s, g.w_m, g.w_h, size_t(kWTotal)));
+  for (float* p : {g.grid_g, g.grid_am, g.grid_av}) HIP_OR_FAIL(c, hipMemsetAsync(p, 0, g.ngrid * 4, s));
+  for (float* p : {g.w_g, g.w_am, g.w_av}) HIP_OR_FAIL(c, hipMemsetAsync(p, 0, size_t(kWTotal) * 4, s));
+  HIP_OR_FAIL(c, hipStreamSynchronize(s));
+  g.step = 0;
+  return XNH_OK;
+}
+
+int xnh_guide_get_params(xnh_ctx* c, float* grid, float* w) {
+  if (!c || !c->guide.on) return XNH_E_INVALID;
+  GuideState& g = c->guide;
+  if (grid) HIP_OR_FAIL(c, hipMemcpyAsync(grid, g.grid_m, g.ngrid * 4, hipMemcpyDeviceToHost, c->stream));
+  if (w) HIP_OR_FAIL(c, hipMemcpyAsync(w, g.w_m, size_t(kWTotal) * 4, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return XNH_OK;
+}
+
+int xnh_guide_eval(xnh_ctx* c, int32_t n, const float* pos, const float* nrm, float* raw) {
+  if (!c || !c->guide.on || n < 0 || (n > 0 && (!pos || !nrm || !raw))) return XNH_E_INVALID;
+  if (n == 0) return XNH_OK;
+  std::vector<void*> tmp;
+  float *dp, *dn, *dr;
+  int rc = dev_alloc(c, tmp, &dp, size_t(n) * 3);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &dn, size_t(n) * 3);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &dr, size_t(n) * kGOut);
+  hipError_t e = hipSuccess;
+  if (rc == XNH_OK) {
+    e = hipMemcpyAsync(dp, pos, size_t(n) * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, nrm, size_t(n) * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_guide_eval(c->stream, c->guide.net, n, dp, dn, dr);
+    if (e == hipSuccess) e = hipMemcpyAsync(raw, dr, size_t(n) * kGOut * 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  free_all(tmp);
+  if (rc != XNH_OK) return rc;
+  if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("guide eval: ") + hipGetErrorString(e));
+  return XNH_OK;
+}
+
+// One step on device-resident samples (lam already on the device).
+static int guide_step(xnh_ctx* c, int n, const float* dp, const float* dn, const float* dd, const float* dl,
+                      float lam_scale, bool apply) {
+  GuideState& g = c->guide;
+  hipStream_t s = c->stream;
+  HIP_OR_FAIL(c, hipMemsetAsync(g.loss, 0, 4, s));
+  HIP_OR_FAIL(c, launch_guide_train(s, g.net, n, dp, dn, dd, dl, lam_scale, g.grid_g, g.w_g, g.loss));
+  if (apply) {
+    g.step++;
+    const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
+    const float bc1 = 1.f - static_cast<float>(std::pow(double(b1), g.step));
+    const float bc2 = 1.f - static_cast<float>(std::pow(double(b2), g.step));
+    const float lr = g.cfg.learning_rate, gs = 1.f / static_cast<float>(n);
+    HIP_OR_FAIL(c, launch_guide_adam(s, g.grid_m, g.grid_h, g.grid_g, g.grid_am, g.grid_av, g.ngrid, lr, b1, b2, eps,
+                                     bc1, bc2, gs));
+    HIP_OR_FAIL(c, launch_guide_adam(s, g.w_m, g.w_h, g.w_g, g.w_am, g.w_av, size_t(kWTotal), lr, b1, b2, eps, bc1,
+                                     bc2, gs));
+  }
+  return XNH_OK;
+}
+
+int xnh_guide_train(xnh_ctx* c, int32_t n, const float* pos, const float* nrm, const float* dir, const float* lam,
+                    int32_t apply, float* loss_out, float* grid_grad, float* w_grad) {
+  if (!c || !c->guide.on || n <= 0 || !pos || !nrm || !dir || !lam) return XNH_E_INVALID;
+  double lsum = 0.0;
+  for (int i = 0; i < n; ++i) lsum += lam[i];
+  if (!(lsum > 0.0)) return fail(c, XNH_E_INVALID, "training weights sum to zero");
+  const float lam_scale = static_cast<float>(double(n) / lsum);
+  GuideState& g = c->guide;
+  std::vector<void*> tmp;
+  float *dp, *dn, *dd, *dl;
+  int rc = dev_alloc(c, tmp, &dp, size_t(n) * 3);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &dn, size_t(n) * 3);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &dd, size_t(n) * 3);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &dl, size_t(n));
+  if (rc != XNH_OK) {
+    free_all(tmp);
+    return rc;
+  }
+  hipStream_t s = c->stream;
+  hipError_t e = hipMemcpyAsync(dp, pos, size_t(n) * 12, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dn, nrm, size_t(n) * 12, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dd, dir, size_t(n) * 12, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(dl, lam, size_t(n) * 4, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    free_all(tmp);
+    return fail(c, XNH_E_HIP, "guide train upload");
+  }
+  rc = guide_step(c, n, dp, dn, dd, dl, lam_scale, apply != 0);
+  float loss = 0.f;
+  if (rc == XNH_OK) {
+    e = hipMemcpyAsync(&loss, g.loss, 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && !apply && grid_grad) e = hipMemcpyAsync(grid_grad, g.grid_g, g.ngrid * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && !apply && w_grad) e = hipMemcpyAsync(w_grad, g.w_g, size_t(kWTotal) * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && !apply) e = hipMemsetAsync(g.grid_g, 0, g.ngrid * 4, s);
+    if (e == hipSuccess && !apply) e = hipMemsetAsync(g.w_g, 0, size_t(kWTotal) * 4, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) rc = fail(c, XNH_E_HIP, std::string("guide train: ") + hipGetErrorString(e));
+  }
+  free_all(tmp);
+  if (loss_out) *loss_out = loss / static_cast<float>(n);
+  return rc;
 }
 
 static int trace_common(xnh_ctx* c, const xnh_ray* rays, int32_t n, bool any, void* out) {

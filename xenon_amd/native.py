@@ -111,6 +111,12 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class GuideConfig(C.Structure):
+    _fields_ = [("t_log2", C.c_int32), ("learning_rate", C.c_float), ("guide_fraction", C.c_float),
+                ("seed", C.c_uint32), ("bounds_lo", C.c_float * 3), ("bounds_hi", C.c_float * 3),
+                ("train_batch", C.c_int32), ("record_paths", C.c_int32)]
+
+
 _vp = C.c_void_p
 SIGNATURES = {
     "xnh_create": (C.c_int, [C.POINTER(Config), C.POINTER(_vp)]),
@@ -139,6 +145,12 @@ SIGNATURES = {
     "xnh_scene_build": (C.c_int, [_vp, C.c_int32]),
     "xnh_scene_flat": (C.c_int, [_vp, C.POINTER(FlatScene)]),
     "xnh_camera_look_at": (C.c_int, [_fp, _fp, _fp, C.c_float, C.c_float, C.POINTER(Camera)]),
+    "xnh_guide_enable": (C.c_int, [_vp, C.POINTER(GuideConfig)]),
+    "xnh_guide_disable": (C.c_int, [_vp]),
+    "xnh_guide_set_params": (C.c_int, [_vp, _fp, _fp]),
+    "xnh_guide_get_params": (C.c_int, [_vp, _fp, _fp]),
+    "xnh_guide_eval": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp]),
+    "xnh_guide_train": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp, C.c_int32, _fp, _fp, _fp]),
 }
 
 _lib = None
