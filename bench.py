@@ -81,7 +81,7 @@ def cpu_baseline(desc, budget_s: float):
     t0 = time.perf_counter()
     osc.render_average(1) if kind == "reference" else osc.render_sum(1, 1, threads)
     one = time.perf_counter() - t0
-    n = max(1, min(16, int(budget_s / max(one, 1e-3))))
+    n = max(1, min(512, int(budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
     if kind == "reference":
         osc.render_average(n)
@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
+                         "for scaling HBM counter readings) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the renderer shards by "
@@ -115,6 +118,17 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        n = args.copy_calibration << 18   # float32 elements
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                          "launches": 5}), flush=True)
+        return
 
     from xenon_amd import HipRenderer, HipScene, scenes
 

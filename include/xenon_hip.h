@@ -242,6 +242,11 @@ int xnh_guide_disable(xnh_ctx* ctx);
 int xnh_guide_set_params(xnh_ctx* ctx, const float* grid, const float* weights);
 int xnh_guide_get_params(xnh_ctx* ctx, float* grid, float* weights);
 int xnh_guide_eval(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm, float* raw_out);
+/* Mixture queries: with u (3 per point) non-null, draws a direction per
+ * point from its mixture into dir; then writes the mixture pdf (per
+ * steradian) at dir into pdf. With u null, dir is an input. */
+int xnh_guide_query(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm, const float* u, float* dir,
+                    float* pdf);
 /* One optimisation step on a host batch: loss -lambda_i log q(dir_i) with
  * lambda scaled to mean 1. With apply == 0 only gradients are computed and,
  * when grid_grad / w_grad are non-null, returned (sums over the batch of the

@@ -35,6 +35,7 @@ N_HID = 64
 N_OUT = 48         # 8 lobes x 5 raw outputs, padded
 N_LOBES = 8
 KAPPA_LOG_MIN, KAPPA_LOG_MAX = -4.0, 8.0
+GRAD_CLIP = 1024.0  # per-sample output-gradient clip (keeps FP16 back-propagation finite)
 PRIMES = (1, 2654435761, 805459861)
 
 
@@ -94,8 +95,21 @@ class GuideParams:
                   for d in dims]
 
 
+POS_MAX = np.float32(0.99999994)  # largest float below 1
+
+
+def unit_pos(p, lo, hi):
+    """World positions -> the unit cube of the grid, clamped to [0, POS_MAX]."""
+    lo = np.asarray(lo, np.float32)
+    e = np.asarray(hi, np.float32) - lo
+    inv = np.where(e > 0, np.float32(1) / np.where(e > 0, e, 1).astype(np.float32), np.float32(1)).astype(np.float32)
+    q = ((np.asarray(p, np.float32) - lo) * inv).astype(np.float32)
+    return np.clip(q, np.float32(0), POS_MAX).astype(np.float32)
+
+
 def encode(P, p01, n):
     """Returns (enc (N,48) fp16-rounded, per-level corner slots and weights)."""
+    p01 = np.clip(np.asarray(p01, np.float32), np.float32(0), POS_MAX)
     N = p01.shape[0]
     feats = np.zeros((N, L_LEVELS * F_FEAT), np.float32)
     slots, weights = [], []
@@ -177,6 +191,7 @@ def loss_and_grad_out(out, w, lam):
     G[:, :, 3] = -lam[:, None] * r * dlogv_dk * kappa * inside
     tang = w[:, None, :] - dots[:, :, None] * mu
     G[:, :, 0:3] = -lam[:, None, None] * r[:, :, None] * kappa[:, :, None] * tang / nm[:, :, None]
+    np.clip(g, -GRAD_CLIP, GRAD_CLIP, out=g)
     return loss.astype(np.float32), g
 
 

@@ -6,6 +6,8 @@
 //           {lo.x, lo.y, lo.z, hi.x} {hi.y, hi.z, child, count}
 //           interior: child = global index of the left child (right = +1);
 //           leaf: child = global triangle slot of its first triangle.
+//           Each mesh starts at an odd global index, so every child pair
+//           (left index odd within the mesh) is 64-byte aligned.
 //   tris    3 x float4 per leaf slot, in leaf order:
 //           {v0.xyz, e1.x} {e1.yz, e2.xy} {e2.z, tri, 0, 0}
 //           (e1 = v1 - v0, e2 = v2 - v0, computed once on the host: the same
@@ -15,7 +17,8 @@
 //           node's longest axis, the traversal order key of tlas.cpp:231-241
 //   inst    4 x float4 per instance: world_to_local columns c0..c3 packed
 //           {c0.xyz, c1.x} {c1.yz, c2.xy} {c2.z, c3.xyz} and
-//           {blas root node, instance_id, tri_base | flat<<31, light map or -1}
+//           {blas root entry, instance_id, tri_base | flat<<31, light map or -1}
+//           (the root's stack entry, see kEntShift; kNoRoot for an empty mesh)
 //   vidx    3 x int per global triangle (global vertex indices)
 //   vnrm    float4 per global vertex (xyz normal)
 //   tmat    material id per global triangle
@@ -33,6 +36,11 @@ namespace xnh {
 
 constexpr int kMaxStack = 128;          // TLAS 64 + BLAS 64, the reference's two stacks
 constexpr uint32_t kFlatBit = 0x80000000u;
+// BLAS stack entries: count << kEntShift | index (see dtrace.h)
+constexpr uint32_t kEntShift = 28;
+constexpr uint32_t kEntIndexMask = (1u << kEntShift) - 1u;
+constexpr uint32_t kEntBigLeaf = 15;
+constexpr uint32_t kNoRoot = 0xFFFFFFFFu;
 
 struct DevSky {
   float zenith[3], horizon[3], ground[3];
@@ -110,11 +118,18 @@ struct WaveBuffers {
   unsigned long long* stats;  // see Stat
   uint32_t* spill;  // traversal stack entries beyond the LDS part
   int32_t capacity;
+  // path guiding (null / 0 when off)
+  float4* mix;      // [kMixVecs][capacity] mixture per queue entry (k_guide_infer)
+  float4* rec;      // [4][rec_cap] training records, see guide.h
+  uint32_t rec_cap;
+  uint32_t rec_every;   // a path records its vertices when hash(slot, batch) % rec_every == 0
+  float guide_alpha;    // one-sample MIS probability of sampling the guide; 0 = off
 };
 
 enum Ctr : int {
   kBatch = 0,                            // batch being rendered (advanced on the device)
   kOverflow = 1,                         // lanes whose stack exceeded kMaxStack
+  kRecCnt = 2,                           // guiding training records of the batch
   kCnt = 16,                             // [queue][chunk] live paths
   kShCnt = kCnt + 2 * kMaxChunks,        // [chunk] shadow rays
   kCursor = kShCnt + kMaxChunks,         // per-launch work cursors of a batch

@@ -12,6 +12,13 @@
 //     longest axis; instances transform the ray without normalising it so t
 //     is shared by both spaces (geometry/transform.h:70-77).
 //
+// BLAS stack entries carry what the walk needs about a node without loading
+// it: the node's child index and triangle count, read from the node record
+// when its parent pair-tests it (blas_entry). Bits 28..31 hold the count
+// (0 = interior: low bits = the left child of the pair; 1..14 = leaf: low
+// bits = its first triangle slot; 15 = large leaf: low bits = the node
+// index, whose record is loaded on pop). Node and slot indices are < 2^28.
+//
 // Stack: the top `lds_depth` entries of each lane live in LDS, laid out
 // entry-major ([entry][lane]) so lanes at equal depth hit distinct banks;
 // deeper entries spill, in order, to a per-lane HBM region. The BLAS walk of
@@ -89,67 +96,10 @@ __device__ __forceinline__ int pair_test(float4 la, float4 lb, float4 ra, float4
   return (ln <= lf ? 1 : 0) | (rn <= rf ? 2 : 0);
 }
 
-// BLAS walk for one instance. Closest: updates tbest/hit on closer (or
-// equal) hits. Any: returns true at the first accepted triangle.
-template <bool kAny>
-__device__ __forceinline__ bool walk_blas(const DevScene& S, uint32_t root, f3 o, f3 d, float tmin, float& tbest,
-                                          HitRef& hit, TravStack& st, TravCount& c) {
-  if (root == 0xFFFFFFFFu) return false;   // mesh without triangles
-  const f3 inv{1.f / d.x, 1.f / d.y, 1.f / d.z};
-  const int base = st.sp;
-  st.push(root);
-  bool found = false;
-  while (st.sp > base) {
-    const uint32_t ni = st.pop();
-    const float4 nb = S.bnodes[2 * ni + 1];
-    const uint32_t child = __float_as_uint(nb.z), count = __float_as_uint(nb.w);
-    if (count > 0) {
-      for (uint32_t i = 0; i < count; ++i) {
-        float t, u, v;
-        ++c.tri_tests;
-        if (tri_test(S.tris, child + i, o, d, tmin, tbest, t, u, v)) {
-          if (kAny) {
-            st.sp = base;
-            return true;
-          }
-          tbest = t;
-          hit.t = t;
-          hit.u = u;
-          hit.v = v;
-          hit.slot = child + i;
-          found = true;
-        }
-      }
-      continue;
-    }
-    const float4 la = S.bnodes[2 * child], lb = S.bnodes[2 * child + 1];
-    const float4 ra = S.bnodes[2 * child + 2], rb = S.bnodes[2 * child + 3];
-    float tn0, tn1;
-    ++c.interior;
-    c.node_tests += 2;
-    const int m = pair_test(la, lb, ra, rb, inv, o, tmin, tbest, tn0, tn1);
-    if (kAny) {
-      if (m & 2) st.push(child + 1);
-      if (m & 1) st.push(child);
-    } else {
-      if (m == 3) {
-        if (tn0 < tn1) {
-          st.push(child + 1);
-          st.push(child);
-        } else {
-          st.push(child);
-          st.push(child + 1);
-        }
-      } else if (m) {
-        st.push((m & 1) ? child : child + 1);
-      }
-    }
-    if (st.overflow) {
-      st.sp = base;
-      return found;
-    }
-  }
-  return found;
+__device__ __forceinline__ uint32_t blas_entry(uint32_t node, uint32_t child, uint32_t count) {
+  if (count == 0) return child;
+  if (count < kEntBigLeaf) return (count << kEntShift) | child;
+  return (kEntBigLeaf << kEntShift) | node;
 }
 
 // Instance: world_to_local columns and metadata (see dev_types.h).
@@ -174,52 +124,148 @@ __device__ __forceinline__ InstRec load_inst(const float4* inst, int i) {
   return r;
 }
 
-// Scene walk: TLAS with nested BLAS walks. Closest hit starts from
+// Scene walk as a resumable state machine: one step pops one stack entry
+// (a TLAS node, an instance, or a BLAS node pair / leaf), so a persistent
+// kernel can retire a finished ray and start the next one in the same lane
+// while the rest of the wave keeps walking. The visiting order, and hence
+// every result and work counter, equals a recursive TLAS walk with nested
+// BLAS walks: a TLAS leaf pushes its instances (tagged with kInstTag) in
+// reverse so they are walked first to last; an instance's BLAS entries sit
+// above `blas_base` on the shared stack. Closest hit starts from
 // t_best = +inf and ignores tmax, as Scene::intersect does with a default
-// HitRecord; any hit uses [tmin, tmax].
+// HitRecord; any hit uses [tmin, tmax]. A stack overflow ends the ray with
+// what it has found so far (counted by the caller).
+constexpr uint32_t kInstTag = 0x80000000u;
+
 template <bool kAny>
-__device__ __forceinline__ bool walk_scene(const DevScene& S, f3 o, f3 d, float tmin, float tmax, HitRef& hit,
-                                           TravStack& st, TravCount& c) {
-  hit.inst = -1;
-  if (S.num_tnodes == 0) return false;
-  const f3 inv{1.f / d.x, 1.f / d.y, 1.f / d.z};
-  const int sx = d.x < 0.f, sy = d.y < 0.f, sz = d.z < 0.f;
-  float tbest = kAny ? tmax : __builtin_huge_valf();
-  st.sp = 0;
-  st.push(0u);
-  bool found = false;
-  while (st.sp > 0) {
-    const uint32_t ni = st.pop();
+struct Walker {
+  f3 o, d, inv;       // world ray
+  f3 lo, ld, linv;    // instance-space ray of the BLAS walk in progress
+  float tmin, tbest;
+  int blas_base;      // stack depth under the current BLAS walk; -1 in the TLAS
+  int inst;           // instance of the BLAS walk
+  bool inst_found, found;
+  HitRef hit;
+
+  __device__ __forceinline__ void begin(const DevScene& S, f3 o_, f3 d_, float tmin_, float tmax_, TravStack& st) {
+    o = o_;
+    d = d_;
+    inv = f3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+    tmin = tmin_;
+    tbest = kAny ? tmax_ : __builtin_huge_valf();
+    blas_base = -1;
+    inst = -1;
+    inst_found = false;
+    found = false;
+    hit = HitRef{0.f, 0.f, 0.f, 0u, -1};
+    st.sp = 0;
+    st.overflow = false;
+    if (S.num_tnodes > 0) st.push(0u);
+  }
+
+  __device__ __forceinline__ void end_instance() {
+    if (!kAny && inst_found) {
+      hit.inst = inst;
+      found = true;
+    }
+    blas_base = -1;
+  }
+
+  // Advances by one stack entry; returns true once the ray is finished
+  // (closest: `found` / `hit`; any: `found` = occluded).
+  __device__ __forceinline__ bool step(const DevScene& S, TravStack& st, TravCount& c) {
+    if (blas_base >= 0 && st.sp == blas_base) end_instance();
+    if (blas_base >= 0) {
+      const uint32_t e = st.pop();
+      uint32_t count = e >> kEntShift, child = e & kEntIndexMask;
+      if (count == kEntBigLeaf) {
+        const float4 nb = S.bnodes[2 * child + 1];
+        child = __float_as_uint(nb.z);
+        count = __float_as_uint(nb.w);
+      }
+      if (count > 0) {
+        for (uint32_t i = 0; i < count; ++i) {
+          float t, u, v;
+          ++c.tri_tests;
+          if (tri_test(S.tris, child + i, lo, ld, tmin, tbest, t, u, v)) {
+            if (kAny) {
+              found = true;
+              return true;
+            }
+            tbest = t;
+            hit.t = t;
+            hit.u = u;
+            hit.v = v;
+            hit.slot = child + i;
+            inst_found = true;
+          }
+        }
+        return false;
+      }
+      const float4 la = S.bnodes[2 * child], lb = S.bnodes[2 * child + 1];
+      const float4 ra = S.bnodes[2 * child + 2], rb = S.bnodes[2 * child + 3];
+      float tn0, tn1;
+      ++c.interior;
+      c.node_tests += 2;
+      const int m = pair_test(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1);
+      const uint32_t el = blas_entry(child, __float_as_uint(lb.z), __float_as_uint(lb.w));
+      const uint32_t er = blas_entry(child + 1, __float_as_uint(rb.z), __float_as_uint(rb.w));
+      if (kAny) {
+        if (m & 2) st.push(er);
+        if (m & 1) st.push(el);
+      } else if (m == 3) {
+        if (tn0 < tn1) {
+          st.push(er);
+          st.push(el);
+        } else {
+          st.push(el);
+          st.push(er);
+        }
+      } else if (m) {
+        st.push((m & 1) ? el : er);
+      }
+      if (st.overflow) {
+        end_instance();
+        return true;
+      }
+      return false;
+    }
+    if (st.sp == 0) return true;
+    const uint32_t e = st.pop();
+    if (e & kInstTag) {
+      const int ii = static_cast<int>(e & ~kInstTag);
+      const InstRec in = load_inst(S.inst, ii);
+      ++c.inst_visits;
+      lo = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
+      ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
+      linv = f3{1.f / ld.x, 1.f / ld.y, 1.f / ld.z};
+      inst = ii;
+      inst_found = false;
+      blas_base = st.sp;
+      if (in.root != kNoRoot) st.push(in.root);
+      if (st.overflow) {
+        end_instance();
+        return true;
+      }
+      return false;
+    }
+    const uint32_t ni = e;
     const float4 na = S.tnodes[3 * ni], nb = S.tnodes[3 * ni + 1];
     ++c.node_tests;
     // sign-selected slab (geometry/aabb.h:59-71): near/far by direction sign
+    const bool sx = d.x < 0.f, sy = d.y < 0.f, sz = d.z < 0.f;
     const float bx0 = sx ? na.w : na.x, bx1 = sx ? na.x : na.w;
     const float by0 = sy ? nb.x : na.y, by1 = sy ? na.y : nb.x;
     const float bz0 = sz ? nb.y : na.z, bz1 = sz ? na.z : nb.y;
     const float txn = (bx0 - o.x) * inv.x, txf = (bx1 - o.x) * inv.x;
     const float tyn = (by0 - o.y) * inv.y, tyf = (by1 - o.y) * inv.y;
     const float tzn = (bz0 - o.z) * inv.z, tzf = (bz1 - o.z) * inv.z;
-    const float lo = smax(smax(smax(tmin, txn), tyn), tzn);
-    const float hi = smin(smin(smin(tbest, txf), tyf), tzf);
-    if (!(lo <= hi)) continue;
+    const float lo_t = smax(smax(smax(tmin, txn), tyn), tzn);
+    const float hi_t = smin(smin(smin(tbest, txf), tyf), tzf);
+    if (!(lo_t <= hi_t)) return false;
     const uint32_t child = __float_as_uint(nb.z), count = __float_as_uint(nb.w);
     if (count > 0) {
-      for (uint32_t k = 0; k < count; ++k) {
-        const int ii = static_cast<int>(S.torder[child + k]);
-        const InstRec in = load_inst(S.inst, ii);
-        ++c.inst_visits;
-        const f3 lo_ = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
-        const f3 ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
-        if (kAny) {
-          if (walk_blas<true>(S, in.root, lo_, ld, tmin, tbest, hit, st, c)) return true;
-        } else {
-          if (walk_blas<false>(S, in.root, lo_, ld, tmin, tbest, hit, st, c)) {
-            hit.inst = ii;
-            found = true;
-          }
-        }
-        if (st.overflow) return found;
-      }
+      for (uint32_t k = count; k-- > 0;) st.push(kInstTag | S.torder[child + k]);
     } else {
       ++c.interior;
       const float4 nc = S.tnodes[3 * ni + 2];
@@ -237,10 +283,21 @@ __device__ __forceinline__ bool walk_scene(const DevScene& S, f3 o, f3 d, float 
         st.push(child);
         st.push(child + 1);
       }
-      if (st.overflow) return found;
     }
+    return st.overflow;
   }
-  return found;
+};
+
+// Whole walk of one ray (the ray-batch API).
+template <bool kAny>
+__device__ __forceinline__ bool walk_scene(const DevScene& S, f3 o, f3 d, float tmin, float tmax, HitRef& hit,
+                                           TravStack& st, TravCount& c) {
+  Walker<kAny> w;
+  w.begin(S, o, d, tmin, tmax, st);
+  while (!w.step(S, st, c)) {
+  }
+  hit = w.hit;
+  return w.found;
 }
 
 }  // namespace xnh

@@ -207,4 +207,66 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u
   return fr.world(f3{st * cosf(phi), st * sinf(phi), wz});
 }
 
+// ---- stored mixtures (one per path-queue entry) --------------------------------
+//
+// mix[k * C + i], k = 0..11, for queue entry i of capacity C:
+//   k = 0..7   {mu_j.xyz, kappa_j}
+//   k = 8, 9   {pi_0..3}, {pi_4..7}
+//   k = 10, 11 pi_j * C(kappa_j), the pdf weight of lobe j
+constexpr int kMixVecs = 12;
+
+__device__ __forceinline__ void store_mixture(float4* mix, uint32_t C, uint32_t i, const Mixture& m) {
+  float w[kGLobes];
+  for (int j = 0; j < kGLobes; ++j) {
+    mix[j * C + i] = make_float4(m.mu[j].x, m.mu[j].y, m.mu[j].z, m.kappa[j]);
+    w[j] = m.pi[j] * vmf_norm(m.kappa[j]);
+  }
+  mix[8 * C + i] = make_float4(m.pi[0], m.pi[1], m.pi[2], m.pi[3]);
+  mix[9 * C + i] = make_float4(m.pi[4], m.pi[5], m.pi[6], m.pi[7]);
+  mix[10 * C + i] = make_float4(w[0], w[1], w[2], w[3]);
+  mix[11 * C + i] = make_float4(w[4], w[5], w[6], w[7]);
+}
+
+__device__ __forceinline__ float lane4(float4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }
+
+// Mixture pdf (per steradian) of the stored mixture at unit direction w.
+__device__ __forceinline__ float stored_pdf(const float4* mix, uint32_t C, uint32_t i, f3 w) {
+  const float4 wa = mix[10 * C + i], wb = mix[11 * C + i];
+  float q = 0.f;
+  for (int j = 0; j < kGLobes; ++j) {
+    const float4 l = mix[j * C + i];
+    q += lane4(j < 4 ? wa : wb, j & 3) * expf(l.w * (l.x * w.x + l.y * w.y + l.z * w.z - 1.f));
+  }
+  return q;
+}
+
+// Draws a direction from the stored mixture (lobe by CDF on u0, then
+// Jakob's vMF inverse CDF on u1, u2 in the lobe's frame).
+__device__ __forceinline__ f3 stored_sample(const float4* mix, uint32_t C, uint32_t i, float u0, float u1, float u2) {
+  const float4 pa = mix[8 * C + i], pb = mix[9 * C + i];
+  int j = 0;
+  float cdf = pa.x;
+  while (j < kGLobes - 1 && cdf < u0) {
+    ++j;
+    cdf += lane4(j < 4 ? pa : pb, j & 3);
+  }
+  const float4 l = mix[j * C + i];
+  const float k = l.w;
+  const float wz = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * k)) / k;
+  const float st = sqrtf(fmaxf(0.f, 1.f - wz * wz));
+  const float phi = 2.f * kGuidePi * u2;
+  const Frame fr(f3{l.x, l.y, l.z});
+  return fr.world(f3{st * cosf(phi), st * sinf(phi), wz});
+}
+
+// ---- training records -----------------------------------------------------------
+//
+// rec[k * R + r], k = 0..3, for record r of capacity R (one per guided
+// vertex of a recording path):
+//   {pos, T.r} {normal, T.g} {dir, T.b} {prefix.rgb, slot bits}
+// T = path throughput at the vertex (after roulette); prefix = the path's
+// radiance sum when its continuation reached the next vertex (written
+// there), so the radiance gathered after the vertex is total - prefix.
+constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
+
 }  // namespace xnh

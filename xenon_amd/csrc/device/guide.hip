@@ -9,8 +9,11 @@
 //   softmax logit a_j:  -lambda (r_j - pi_j)
 //   log kappa_j:        -lambda r_j kappa_j (1/k - 2/(e^{2k} - 1) + mu.w - 1)
 //   raw mean m_j:       -lambda r_j kappa_j (w - (mu.w) mu) / |m_j|
-// with responsibilities r_j = pi_j v_j(w) / q(w). Gradients are summed over
-// the batch (FP32 atomics); the caller divides by the batch size in Adam.
+// with responsibilities r_j = pi_j v_j(w) / q(w). Each per-sample output
+// gradient is clipped to +-kGradClip so that it and the back-propagated
+// gradients stay inside FP16 range (outlier samples with large lambda);
+// back-propagated FP16 values saturate at +-kHalfMax. Gradients are summed
+// over the batch (FP32 atomics); the caller divides by the batch size in Adam.
 #include <hip/hip_runtime.h>
 
 #include "guide.h"
@@ -19,6 +22,9 @@
 namespace xnh {
 
 namespace {
+
+constexpr float kGradClip = 1024.f;
+constexpr float kHalfMax = 65504.f;
 
 __device__ __forceinline__ int glane() { return static_cast<int>(threadIdx.x & 63u); }
 
@@ -59,6 +65,7 @@ __device__ __forceinline__ float loss_grad(const float* raw, f3 w, float lam, fl
     g[5 * j + 3] = -lam * r * dlogv * k * inside;
     g[5 * j + 4] = -lam * (r - m.pi[j]);
   }
+  for (int k = 0; k < kGLobes * 5; ++k) g[k] = fminf(fmaxf(g[k], -kGradClip), kGradClip);
   return -lam * logf(q);
 }
 
@@ -87,7 +94,7 @@ __device__ __forceinline__ void input_grad(const _Float16* G, int N, const _Floa
         const int r = 16 * rt + 4 * (lane >> 4) + j, col = 16 * ct + (lane & 15);
         float v = acc[j];
         if (H && !(static_cast<float>(H[r * K + col]) > 0.f)) v = 0.f;
-        if (Gout) Gout[r * K + col] = static_cast<_Float16>(v);
+        if (Gout) Gout[r * K + col] = static_cast<_Float16>(fminf(fmaxf(v, -kHalfMax), kHalfMax));
         else Gout32[r * K + col] = v;
       }
     }
@@ -117,6 +124,35 @@ __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const fl
   mlp_forward(net, s.X, s.H1, s.H2, s.H3, s.F);
   if (ok)
     for (int k = 0; k < kGOut; ++k) raw[static_cast<size_t>(i) * kGOut + k] = s.F[lane * kGOut + k];
+}
+
+// Mixture queries for n points: with u non-null, draws one direction per
+// point from its mixture with (u0, u1, u2) = u[3i..3i+2] and writes it to
+// dir; then writes the mixture pdf at dir[i] (drawn or given) to pdf.
+__global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const float* pos, const float* nrm,
+                                                    const float* u, float* dir, float* pdf) {
+  __shared__ TrainLds s;
+  const int lane = glane();
+  const int i = static_cast<int>(blockIdx.x) * 64 + lane;
+  const bool ok = i < n;
+  const f3 p = ok ? f3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]} : f3{net.lo[0], net.lo[1], net.lo[2]};
+  const f3 nn = ok ? f3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]} : f3{0.f, 0.f, 1.f};
+  guide_encode(net, p, nn, s.X + lane * kGIn);
+  __syncthreads();
+  mlp_forward(net, s.X, s.H1, s.H2, s.H3, s.F);
+  if (!ok) return;
+  Mixture m;
+  decode_mixture(s.F + lane * kGOut, m);
+  f3 w;
+  if (u) {
+    w = mixture_sample(m, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+    dir[3 * i] = w.x;
+    dir[3 * i + 1] = w.y;
+    dir[3 * i + 2] = w.z;
+  } else {
+    w = f3{dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+  }
+  pdf[i] = mixture_pdf(m, w);
 }
 
 // Forward + loss + backward over n training samples; accumulates the
@@ -200,6 +236,13 @@ hipError_t launch_guide_eval(hipStream_t s, const GuideNet& net, int n, const fl
                              float* raw) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_guide_eval, dim3((n + 63) / 64), dim3(64), 0, s, net, n, pos, nrm, raw);
+  return hipGetLastError();
+}
+
+hipError_t launch_guide_query(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
+                              const float* u, float* dir, float* pdf) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_guide_query, dim3((n + 63) / 64), dim3(64), 0, s, net, n, pos, nrm, u, dir, pdf);
   return hipGetLastError();
 }
 

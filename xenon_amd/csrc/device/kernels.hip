@@ -246,9 +246,82 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
 }
 
 // ---------------------------------------------------------------------------
-// closest: persistent closest-hit traversal of queue qi. Each wave takes
-// 64-entry units from a device cursor; a unit lies in one chunk, so the
-// job (and its scene) is wave-uniform.
+// Persistent traversal of one queue with per-lane refill. Waves take
+// 64-entry units from a device cursor into a wave-local pool; a lane whose
+// ray is finished takes the next pool entry at once, so a wave keeps all
+// lanes busy until the queue runs dry instead of waiting for its slowest
+// ray. A unit lies in one chunk; the rays in flight in a wave always share
+// one job (and scene): a unit of another job waits until the wave drains.
+// seg: queue entries per chunk sample slot (1 path queue, 2 shadow queue).
+// Must be reached by every lane of the wave.
+// ---------------------------------------------------------------------------
+template <bool kAny, class Load, class Finish>
+__device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchView& v, uint32_t* cursor,
+                                            uint32_t seg, TravStack& st, TravCount& c, uint32_t& rays,
+                                            bool& overflow, Load load, Finish finish) {
+  constexpr uint32_t kIdle = 0xFFFFFFFFu;
+  const uint32_t total = v.prefix[v.nch];
+  const int lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t my = kIdle;
+  Walker<kAny> w;
+  uint32_t pnext = 0, pend = 0;   // wave-uniform pool of offsets in chunk pci
+  int pci = 0;
+  int held = -1;                  // fetched unit of another job, waiting for the wave to drain
+  bool exhausted = false;
+  for (;;) {
+    const unsigned long long act = __ballot(my != kIdle);
+    if (act != ~0ull) {
+      if (pnext >= pend) {
+        int unit = -1;
+        if (held >= 0) {
+          if (act == 0ull) {
+            unit = held;
+            held = -1;
+          }
+        } else if (!exhausted) {
+          uint32_t u = 0;
+          if (lane == 0) u = atomicAdd(cursor, 1u);
+          u = static_cast<uint32_t>(uniform(static_cast<int>(__shfl(u, 0))));
+          if (u >= total) exhausted = true;
+          else if (act != 0ull && v.ch[find_chunk(v, u)].job != v.ch[pci].job) held = static_cast<int>(u);
+          else unit = static_cast<int>(u);
+        }
+        if (unit >= 0) {
+          pci = uniform(find_chunk(v, static_cast<uint32_t>(unit)));
+          pnext = (static_cast<uint32_t>(unit) - v.prefix[pci]) * 64u;
+          pend = min(pnext + 64u, v.cnt[pci]);
+        }
+      }
+      if (pnext < pend) {
+        const unsigned long long idle = ~act;
+        const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
+        const DevScene& S = B.jobs[uniform(v.ch[pci].job)].S;
+        if (my == kIdle && pnext + rank < pend) {
+          my = seg * static_cast<uint32_t>(v.ch[pci].base) + pnext + rank;
+          f3 o, d;
+          float tmin, tmax;
+          load(my, o, d, tmin, tmax);
+          w.begin(S, o, d, tmin, tmax, st);
+        }
+        pnext = min(pend, pnext + static_cast<uint32_t>(__popcll(idle)));
+      }
+    }
+    if (__ballot(my != kIdle) == 0ull) break;
+    if (my != kIdle) {
+      const DevScene& S = B.jobs[uniform(v.ch[pci].job)].S;
+      if (w.step(S, st, c)) {
+        overflow |= st.overflow;
+        finish(my, w);
+        ++rays;
+        my = kIdle;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// closest: persistent closest-hit traversal of queue qi.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int cursor, int lds_depth) {
   extern __shared__ uint32_t lds[];
@@ -259,34 +332,24 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
     B.counters[kCnt + (1 - qi) * kMaxChunks + threadIdx.x] = 0u;
     B.counters[kShCnt + threadIdx.x] = 0u;
   }
-  const uint32_t total = v.prefix[v.nch];
   const PathQueue q = B.q[qi];
   TravStack st = make_stack(lds, B.spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
-  for (;;) {
-    uint32_t unit = 0;
-    if (lane_id() == 0) unit = atomicAdd(&B.counters[kCursor + cursor], 1u);
-    unit = static_cast<uint32_t>(uniform(static_cast<int>(__shfl(unit, 0))));
-    if (unit >= total) break;
-    const int ci = uniform(find_chunk(v, unit));
-    const DevChunk& ck = v.ch[ci];
-    const uint32_t off = (unit - v.prefix[ci]) * 64u + static_cast<uint32_t>(lane_id());
-    if (off < v.cnt[ci]) {
-      const DevScene& S = B.jobs[uniform(ck.job)].S;
-      const uint32_t i = static_cast<uint32_t>(ck.base) + off;
-      const float4 r0 = q.ray0[i], r1 = q.ray1[i];
-      const f3 o{r0.x, r0.y, r0.z}, d{r0.w, r1.x, r1.y};
-      HitRef h{0.f, 0.f, 0.f, 0u, -1};
-      st.overflow = false;
-      walk_scene<false>(S, o, d, kBounceOffset, __builtin_huge_valf(), h, st, c);
-      overflow |= st.overflow;
-      B.hit[i] = make_float4(h.t, h.u, h.v, __int_as_float(h.inst));
-      B.hit_slot[i] = h.slot;
-      ++rays;
-    }
-  }
+  trace_queue<false>(
+      B, v, &B.counters[kCursor + cursor], 1u, st, c, rays, overflow,
+      [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
+        const float4 r0 = q.ray0[i], r1 = q.ray1[i];
+        o = f3{r0.x, r0.y, r0.z};
+        d = f3{r0.w, r1.x, r1.y};
+        tmin = kBounceOffset;
+        tmax = __builtin_huge_valf();
+      },
+      [&](uint32_t i, const Walker<false>& w) {
+        B.hit[i] = make_float4(w.hit.t, w.hit.u, w.hit.v, __int_as_float(w.found ? w.hit.inst : -1));
+        B.hit_slot[i] = w.hit.slot;
+      });
   flush_counts(B.stats, c, rays, false);
   if (overflow) atomicAdd(&B.counters[kOverflow], 1u);
 }
@@ -498,40 +561,30 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + kShCnt;
   load_batch(v, B, counts, false);
-  const uint32_t total = v.prefix[v.nch];
   TravStack st = make_stack(lds, B.spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
-  for (;;) {
-    uint32_t unit = 0;
-    if (lane_id() == 0) unit = atomicAdd(&B.counters[kCursor + cursor], 1u);
-    unit = static_cast<uint32_t>(uniform(static_cast<int>(__shfl(unit, 0))));
-    if (unit >= total) break;
-    const int ci = uniform(find_chunk(v, unit));
-    const DevChunk& ck = v.ch[ci];
-    const uint32_t off = (unit - v.prefix[ci]) * 64u + static_cast<uint32_t>(lane_id());
-    if (off < v.cnt[ci]) {
-      const DevScene& S = B.jobs[uniform(ck.job)].S;
-      const uint32_t i = 2u * static_cast<uint32_t>(ck.base) + off;
-      const float4 r0 = B.sh0[i], r1 = B.sh1[i];
-      const f3 o{r0.x, r0.y, r0.z}, d{r0.w, r1.x, r1.y};
-      HitRef h{0.f, 0.f, 0.f, 0u, -1};
-      st.overflow = false;
-      const bool occ = walk_scene<true>(S, o, d, kShadowOff, r1.z, h, st, c);
-      overflow |= st.overflow;
-      if (!occ) {
-        const uint32_t k = __float_as_uint(r1.w);
-        const float4 cv = B.sh_c[i];
-        float4 acc = B.l_nee[k];
-        acc.x = acc.x + cv.x;
-        acc.y = acc.y + cv.y;
-        acc.z = acc.z + cv.z;
-        B.l_nee[k] = acc;
-      }
-      ++rays;
-    }
-  }
+  trace_queue<true>(
+      B, v, &B.counters[kCursor + cursor], 2u, st, c, rays, overflow,
+      [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
+        const float4 r0 = B.sh0[i], r1 = B.sh1[i];
+        o = f3{r0.x, r0.y, r0.z};
+        d = f3{r0.w, r1.x, r1.y};
+        tmin = kShadowOff;
+        tmax = r1.z;
+      },
+      [&](uint32_t i, const Walker<true>& w) {
+        if (!w.found) {
+          const uint32_t k = __float_as_uint(B.sh1[i].w);
+          const float4 cv = B.sh_c[i];
+          float4 acc = B.l_nee[k];
+          acc.x = acc.x + cv.x;
+          acc.y = acc.y + cv.y;
+          acc.z = acc.z + cv.z;
+          B.l_nee[k] = acc;
+        }
+      });
   flush_counts(B.stats, c, rays, true);
   if (overflow) atomicAdd(&B.counters[kOverflow], 1u);
 }

@@ -26,6 +26,8 @@ hipError_t launch_api_any(hipStream_t s, const DevScene& S, const float4* rays, 
 
 hipError_t launch_guide_eval(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
                              float* raw);
+hipError_t launch_guide_query(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
+                              const float* u, float* dir, float* pdf);
 hipError_t launch_guide_train(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
                               const float* dir, const float* lam, float lam_scale, float* grid_g, float* w_g,
                               float* loss_sum);

@@ -81,7 +81,7 @@ struct xnh_ctx {
   uint64_t graph_gen = 0;
   int nsamples = 1;               // samples per pixel per batch at the configured resolution
   int block = 256;
-  int lds_depth = 32;
+  int lds_depth = 16;
   int grid_trace = 0;
   int grid_shade = 0;
   float* d_out = nullptr;
@@ -146,6 +146,11 @@ float ibits(int32_t i) {
   std::memcpy(&f, &i, 4);
   return f;
 }
+uint32_t as_uint(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
 
 // Geometric normal of a light triangle, mesh.h:64-68 (host IEEE order).
 void face_normal(const float p0[3], const float p1[3], const float p2[3], float out[3]) {
@@ -172,6 +177,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
     if (me.num_triangles < 0 || me.num_vertices < 0 || me.num_nodes < 0)
       return fail(c, XNH_E_INVALID, "negative mesh count");
     if (me.num_triangles > 0 && me.num_nodes < 1) return fail(c, XNH_E_INVALID, "mesh without BLAS nodes");
+    if ((nn & 1u) == 0) ++nn;   // odd root: child pairs on 64-byte boundaries
     node_base[m] = static_cast<uint32_t>(nn);
     slot_base[m] = static_cast<uint32_t>(ns);
     tri_base[m] = static_cast<uint32_t>(ns);
@@ -180,7 +186,8 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
     ns += static_cast<size_t>(me.num_triangles);
     nv += static_cast<size_t>(me.num_vertices);
   }
-  if (ns >= (1u << 31)) return fail(c, XNH_E_INVALID, "too many triangles");
+  if (ns >= (size_t(1) << kEntShift) || nn >= (size_t(1) << kEntShift))
+    return fail(c, XNH_E_INVALID, "too many triangles or BLAS nodes (limit 2^28)");
   std::vector<float4> bnodes(2 * nn), tris(3 * ns), vnrm(nv);
   std::vector<int32_t> vidx(3 * ns), tmat(ns);
   std::vector<bool> flat(nm, false);
@@ -268,7 +275,13 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
       const uint32_t t = fs->lights[l].tri_idx;
       if (t < static_cast<uint32_t>(ntri) && lmap[lm + t] < 0) lmap[lm + t] = l;
     }
-    const uint32_t root = fs->meshes[in.mesh].num_nodes > 0 ? node_base[in.mesh] : 0xFFFFFFFFu;
+    uint32_t root = kNoRoot;
+    if (fs->meshes[in.mesh].num_nodes > 0) {
+      const uint32_t rn = node_base[in.mesh];
+      const float4 rb = bnodes[2 * rn + 1];
+      const uint32_t rc = as_uint(rb.z), cnt = as_uint(rb.w);
+      root = cnt == 0 ? rc : (cnt < kEntBigLeaf ? (cnt << kEntShift) | rc : (kEntBigLeaf << kEntShift) | rn);
+    }
     inst[4 * i] = f4(w[0], w[1], w[2], w[3]);
     inst[4 * i + 1] = f4(w[4], w[5], w[6], w[7]);
     inst[4 * i + 2] = f4(w[8], w[9], w[10], w[11]);
@@ -632,7 +645,7 @@ int xnh_create(const xnh_config* cfg, xnh_ctx** out) {
     delete c;
     return XNH_E_INVALID;
   }
-  c->lds_depth = cfg->lds_stack > 0 ? std::min(cfg->lds_stack, kMaxStack) : 32;
+  c->lds_depth = cfg->lds_stack > 0 ? std::min(cfg->lds_stack, kMaxStack) : 16;
   c->use_graph = (cfg->flags & XNH_FLAG_NO_GRAPH) == 0;
   const long long npix = static_cast<long long>(cfg->width) * cfg->height;
   if (cfg->spp_per_launch > 0) {
@@ -946,6 +959,35 @@ int xnh_guide_eval(xnh_ctx* c, int32_t n, const float* pos, const float* nrm, fl
   free_all(tmp);
   if (rc != XNH_OK) return rc;
   if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("guide eval: ") + hipGetErrorString(e));
+  return XNH_OK;
+}
+
+int xnh_guide_query(xnh_ctx* c, int32_t n, const float* pos, const float* nrm, const float* u, float* dir,
+                    float* pdf) {
+  if (!c || !c->guide.on || n < 0 || (n > 0 && (!pos || !nrm || !dir || !pdf))) return XNH_E_INVALID;
+  if (n == 0) return XNH_OK;
+  std::vector<void*> tmp;
+  float *dp, *dn, *du = nullptr, *dd, *dq;
+  int rc = dev_alloc(c, tmp, &dp, size_t(n) * 3);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &dn, size_t(n) * 3);
+  if (rc == XNH_OK && u) rc = dev_alloc(c, tmp, &du, size_t(n) * 3);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &dd, size_t(n) * 3);
+  if (rc == XNH_OK) rc = dev_alloc(c, tmp, &dq, size_t(n));
+  hipError_t e = hipSuccess;
+  if (rc == XNH_OK) {
+    hipStream_t s = c->stream;
+    e = hipMemcpyAsync(dp, pos, size_t(n) * 12, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, nrm, size_t(n) * 12, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && u) e = hipMemcpyAsync(du, u, size_t(n) * 12, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !u) e = hipMemcpyAsync(dd, dir, size_t(n) * 12, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = launch_guide_query(s, c->guide.net, n, dp, dn, du, dd, dq);
+    if (e == hipSuccess && u) e = hipMemcpyAsync(dir, dd, size_t(n) * 12, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(pdf, dq, size_t(n) * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+  }
+  free_all(tmp);
+  if (rc != XNH_OK) return rc;
+  if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("guide query: ") + hipGetErrorString(e));
   return XNH_OK;
 }
 

@@ -150,6 +150,7 @@ SIGNATURES = {
     "xnh_guide_set_params": (C.c_int, [_vp, _fp, _fp]),
     "xnh_guide_get_params": (C.c_int, [_vp, _fp, _fp]),
     "xnh_guide_eval": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp]),
+    "xnh_guide_query": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp, _fp]),
     "xnh_guide_train": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp, _fp, C.c_int32, _fp, _fp, _fp]),
 }
 

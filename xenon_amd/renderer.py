@@ -229,6 +229,81 @@ class HipRenderer:
         self._chk(N.lib().xnh_render_jobs(self.h, arr, len(jobs)))
         return (outs, sums) if want_sums else outs
 
+    # -- path guiding -----------------------------------------------------------
+    GUIDE_LEVELS, GUIDE_FEATURES, GUIDE_WEIGHTS = 16, 2, 48 * 64 + 64 * 64 + 64 * 64 + 64 * 48
+
+    def guide_enable(self, bounds_lo, bounds_hi, t_log2: int = 19, learning_rate: float = 5e-3,
+                     guide_fraction: float = 0.5, seed: int = 0, train_batch: int = 0, record_paths: int = 0):
+        """Turns learned path guiding on with fresh parameters; positions
+        inside [bounds_lo, bounds_hi] are mapped to the hash grid."""
+        cfg = N.GuideConfig(int(t_log2), float(learning_rate), float(guide_fraction), int(seed) & 0xFFFFFFFF,
+                            (C.c_float * 3)(*bounds_lo), (C.c_float * 3)(*bounds_hi), int(train_batch),
+                            int(record_paths))
+        self._chk(N.lib().xnh_guide_enable(self.h, C.byref(cfg)))
+        self._guide_t = 1 << int(t_log2 or 19)
+
+    def guide_disable(self):
+        self._chk(N.lib().xnh_guide_disable(self.h))
+
+    def guide_params(self):
+        """FP32 master parameters: (grid (16, T, 2), weights (12288,))."""
+        grid = np.empty((self.GUIDE_LEVELS, self._guide_t, self.GUIDE_FEATURES), np.float32)
+        w = np.empty(self.GUIDE_WEIGHTS, np.float32)
+        self._chk(N.lib().xnh_guide_get_params(self.h, _fp(grid), _fp(w)))
+        return grid, w
+
+    def guide_set_params(self, grid: np.ndarray, weights: np.ndarray):
+        grid = np.ascontiguousarray(grid, np.float32)
+        weights = np.ascontiguousarray(weights, np.float32)
+        if grid.size != self.GUIDE_LEVELS * self._guide_t * self.GUIDE_FEATURES or weights.size != self.GUIDE_WEIGHTS:
+            raise ValueError("guide parameter sizes do not match the enabled configuration")
+        self._chk(N.lib().xnh_guide_set_params(self.h, _fp(grid), _fp(weights)))
+
+    def guide_eval(self, pos: np.ndarray, nrm: np.ndarray) -> np.ndarray:
+        """Raw network outputs, (n, 48)."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        out = np.empty((pos.shape[0], 48), np.float32)
+        self._chk(N.lib().xnh_guide_eval(self.h, pos.shape[0], _fp(pos), _fp(nrm), _fp(out)))
+        return out
+
+    def guide_sample(self, pos, nrm, u):
+        """One mixture sample per point from uniforms u (n, 3): (dir, pdf)."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        u = np.ascontiguousarray(u, np.float32).reshape(-1, 3)
+        d = np.empty_like(pos)
+        pdf = np.empty(pos.shape[0], np.float32)
+        self._chk(N.lib().xnh_guide_query(self.h, pos.shape[0], _fp(pos), _fp(nrm), _fp(u), _fp(d), _fp(pdf)))
+        return d, pdf
+
+    def guide_pdf(self, pos, nrm, dirs):
+        """Mixture pdf (per steradian) of each point at the given directions."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3).copy()
+        pdf = np.empty(pos.shape[0], np.float32)
+        self._chk(N.lib().xnh_guide_query(self.h, pos.shape[0], _fp(pos), _fp(nrm), None, _fp(d), _fp(pdf)))
+        return pdf
+
+    def guide_train(self, pos, nrm, dirs, lam, apply: bool = True, want_grads: bool = False):
+        """One training step on a host batch. Returns the mean scaled loss,
+        and with want_grads (apply False) the summed gradients
+        (grid (16, T, 2), weights (12288,))."""
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        lam = np.ascontiguousarray(lam, np.float32).reshape(-1)
+        loss = C.c_float()
+        gg = gw = None
+        if want_grads:
+            gg = np.empty((self.GUIDE_LEVELS, self._guide_t, self.GUIDE_FEATURES), np.float32)
+            gw = np.empty(self.GUIDE_WEIGHTS, np.float32)
+        self._chk(N.lib().xnh_guide_train(self.h, pos.shape[0], _fp(pos), _fp(nrm), _fp(dirs), _fp(lam),
+                                          1 if apply else 0, C.byref(loss), _fp(gg) if gg is not None else None,
+                                          _fp(gw) if gw is not None else None))
+        return (loss.value, gg, gw) if want_grads else loss.value
+
     # -- ray batches ----------------------------------------------------------
     def trace_closest(self, rays: np.ndarray) -> np.ndarray:
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
