@@ -18,6 +18,14 @@ triangle test and 96 B per instance visit (counted on the device for exactly
 the rays traced); achieved = those bytes / the kernel's summed launch time
 measured with HIP events on the library's stream during the timed steps.
 
+guided: the guided-vs-plain comparison at equal wall-clock time on the
+caustic room (a 499,280-triangle delta-glass sphere in a closed diffuse room
+lit only through a wall opening): a converged plain render (sample indices
+disjoint from the compared runs) is the reference; plain and guided renders
+each get the same wall-clock budget (guided: training after every batch
+included); relMSE = mean over finite pixels and channels of
+(x - r)^2 / (r^2 + 1e-2).
+
 cpu_baseline: the reference's CPU tiled integrator, compiled from its sources
 with its Release flags (oracle/_ref) where present, else the oracle's
 restatement with the same flags; timed on this host over a bounded sample of
@@ -67,6 +75,59 @@ def read_traffic():
     return vals.get("bytes_per_launch")
 
 
+def relmse(x, r):
+    import numpy as np
+    ok = np.isfinite(x) & np.isfinite(r)
+    return float(np.mean((x[ok] - r[ok]) ** 2 / (r[ok] ** 2 + 1e-2)))
+
+
+def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, freq: int):
+    """Plain vs guided relMSE at equal wall-clock on the caustic room."""
+    from xenon_amd import HipRenderer, HipScene, scenes
+
+    desc = scenes.caustic_room(width=width, height=height, freq=freq)
+    hs = HipScene(desc)
+    r = HipRenderer(desc.width, desc.height, desc.max_bounces, desc.min_bounces)
+    r.upload_scene(hs)
+    cam = hs.camera()
+    chunk = 64
+    t0 = time.perf_counter()
+    done = 0
+    while done < ref_spp:
+        n = min(chunk * 16, ref_spp - done)
+        r.render(cam, 10_000_001 + done, n, want_image=False)
+        done += n
+    ref, _ = r.read_accum()
+    ref = ref / ref_spp
+    log(f"guided: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+
+    def run(guided: bool):
+        r.reset()
+        if guided:
+            r.guide_enable()
+        t = time.perf_counter()
+        spp = 0
+        while time.perf_counter() - t < seconds:
+            r.render(cam, 1 + spp, chunk, want_image=False)
+            spp += chunk
+        el = time.perf_counter() - t
+        img, n = r.read_accum()
+        if guided:
+            r.guide_disable()
+        return img / n, n, el
+
+    plain, sp, tp = run(False)
+    guided, sg, tg = run(True)
+    rp, rg = relmse(plain, ref), relmse(guided, ref)
+    log(f"guided: plain {sp} spp relMSE {rp:.5f}; guided {sg} spp relMSE {rg:.5f}")
+    r.close()
+    return {"workload": f"caustic room ({desc.num_tris} tris), {desc.width}x{desc.height}, depth {desc.max_bounces}",
+            "seconds": round(seconds, 2), "reference_spp": ref_spp,
+            "plain": {"spp": sp, "seconds": round(tp, 2), "relmse": round(rp, 6)},
+            "guided": {"spp": sg, "seconds": round(tg, 2), "relmse": round(rg, 6)},
+            "relmse_ratio_plain_over_guided": round(rp / rg, 3) if rg > 0 else None}
+
+
 def cpu_baseline(desc, budget_s: float):
     """Times the CPU integrator on this host; returns the cpu_baseline object."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -108,6 +169,12 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--guided-seconds", type=float, default=1.5,
+                    help="wall-clock budget of each side of the guided-vs-plain comparison (0: skip)")
+    ap.add_argument("--guided-width", type=int, default=320)
+    ap.add_argument("--guided-height", type=int, default=180)
+    ap.add_argument("--guided-ref-spp", type=int, default=65536)
+    ap.add_argument("--guided-freq", type=int, default=158)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "
                          "for scaling HBM counter readings) and exit")
@@ -201,6 +268,9 @@ def main():
         "roofline": roofline,
     }
     r.close()
+    if args.guided_seconds > 0:
+        out["guided"] = guided_comparison(args.guided_seconds, args.guided_width, args.guided_height,
+                                          args.guided_ref_spp, args.guided_freq)
     if not args.no_cpu:
         log("timing the CPU integrator ...")
         cb = cpu_baseline(desc, args.cpu_seconds)

@@ -235,8 +235,15 @@ typedef struct {
   int32_t record_paths;    /* paths per batch that record training samples (0: 2^16) */
 } xnh_guide_config;
 
+/* Turns guided rendering on for this context with fresh parameters. From
+ * then on every render samples exact-Lambertian surfaces by one-sample MIS
+ * of the guide (probability guide_fraction) and the BSDF, and after every
+ * batch trains the network on up to train_batch vertices of about
+ * record_paths paths of that batch (all on the device). */
 int xnh_guide_enable(xnh_ctx* ctx, const xnh_guide_config* cfg);
 int xnh_guide_disable(xnh_ctx* ctx);
+/* 1: train after every batch (default); 0: render with the network frozen. */
+int xnh_guide_set_training(xnh_ctx* ctx, int32_t on);
 /* FP32 master parameters: grid [16][2^t_log2][2], weights 48x64, 64x64,
  * 64x64, 64x48 (row-major [in][out], concatenated). */
 int xnh_guide_set_params(xnh_ctx* ctx, const float* grid, const float* weights);
@@ -253,6 +260,26 @@ int xnh_guide_query(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm,
  * scaled loss); with apply != 0 Adam updates the parameters. */
 int xnh_guide_train(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm, const float* dir, const float* lam,
                     int32_t apply, float* loss_out, float* grid_grad, float* w_grad);
+
+/* ---- Tuning (no counterpart in the reference, whose CUDA backend fixes
+ * 256-thread blocks at cuda_renderer.cu:29) ----
+ * Workgroup size of the traversal kernels (64..256, multiple of 64), LDS
+ * traversal-stack entries per lane (1..128; deeper entries spill to HBM) and
+ * samples per pixel per batch (sets the queue capacity; 0 = auto, about 8M
+ * paths). Changing them reallocates the queues and clears the running
+ * accumulation. */
+typedef struct {
+  int32_t block_size, lds_stack, spp_per_launch;
+  float ms_per_spp;       /* device time of the winner per sample per pixel */
+  int32_t candidates;     /* configurations timed */
+} xnh_tune_result;
+
+int xnh_set_tuning(xnh_ctx* ctx, int32_t block_size, int32_t lds_stack, int32_t spp_per_launch);
+int xnh_get_tuning(const xnh_ctx* ctx, int32_t* block_size, int32_t* lds_stack, int32_t* spp_per_launch);
+/* Times every candidate configuration on `probe_spp` samples of the
+ * uploaded scene through `camera` and keeps the fastest. Clears the
+ * accumulation. */
+int xnh_autotune(xnh_ctx* ctx, const xnh_camera* camera, int32_t probe_spp, xnh_tune_result* out);
 
 /* Copies the raw per-pixel sums (W*H*3) and returns the sample count. */
 int xnh_read_accum(xnh_ctx* ctx, float* sum_out, int32_t* spp_out);

@@ -1,8 +1,11 @@
 """Ray-batch traversal parity: HIP closest/any hit vs the CPU oracle on
 identical rays. Integer results (instance, triangle, material ids, facing,
-occlusion) and the traversal work counters must be bit-identical; float
-results are compared as raw bits and must be identical too, apart from a
-grazing-edge fraction that the tolerance below bounds at 1e-5."""
+occlusion) must be bit-identical; float results are compared as raw bits and
+must be identical too, apart from a grazing-edge fraction that the tolerance
+below bounds at 1e-5. The device walk visits nodes speculatively (a lane
+keeps walking interior nodes while a leaf waits for the wave's leaf phase),
+which can only add work: instance visits must equal the oracle's, node and
+triangle tests may exceed them by a bounded margin, never fall short."""
 import numpy as np
 import pytest
 
@@ -45,6 +48,15 @@ def _compare_closest(hf, hi, hits):
     return id_bad, f_bad, f_dev
 
 
+def _check_work(dev, ref):
+    """Device work counters vs the oracle's recursive walk on the same rays."""
+    ref = [int(x) for x in ref]
+    print(f"  work node/interior/tri/inst: device {dev}, oracle {ref}", flush=True)
+    assert dev[3] == ref[3]
+    for k in range(3):
+        assert ref[k] <= dev[k] <= 1.5 * ref[k] + 64, k
+
+
 def test_closest_bit_exact(pair):
     desc, osc, r = pair
     rays = standard_batch(desc, osc.camera(), 4096, seed=1)
@@ -62,8 +74,7 @@ def test_closest_bit_exact(pair):
         both = (i[:, 0] >= 0) & ~id_bad
         assert np.allclose(f_dev[both, 0], f[both, 0], rtol=T_RTOL, atol=0)
         if frac == 0.0:
-            # identical rays through identical trees: identical work
-            assert [st["node_tests"], st["interior"], st["tri_tests"], st["inst_visits"]] == [int(x) for x in c]
+            _check_work([st["node_tests"], st["interior"], st["tri_tests"], st["inst_visits"]], c)
 
 
 def test_any_bit_exact(pair):
@@ -78,8 +89,8 @@ def test_any_bit_exact(pair):
     print(f"  {desc.name}: {len(srays)} shadow rays, occluded {int(occ_ref.sum())}, mismatches {bad}", flush=True)
     assert bad / len(srays) <= GRAZING_MAX
     if bad == 0:
-        assert [st["shadow_node_tests"], st["shadow_interior"], st["shadow_tri_tests"],
-                st["shadow_inst_visits"]] == [int(x) for x in cnt]
+        _check_work([st["shadow_node_tests"], st["shadow_interior"], st["shadow_tri_tests"],
+                     st["shadow_inst_visits"]], cnt)
 
 
 def test_empty_batch_and_errors(pair):

@@ -41,6 +41,11 @@ constexpr uint32_t kEntShift = 28;
 constexpr uint32_t kEntIndexMask = (1u << kEntShift) - 1u;
 constexpr uint32_t kEntBigLeaf = 15;
 constexpr uint32_t kNoRoot = 0xFFFFFFFFu;
+// Iteration bounds of the traversal loops: far above any legitimate walk
+// (a wave's whole share of a batch takes ~1e5 rounds), they guarantee every
+// wave exits; a wave that reaches one abandons its work and is reported.
+constexpr uint32_t kWaveRoundLimit = 1u << 26;
+constexpr uint32_t kPhaseStepLimit = 1u << 22;
 
 struct DevSky {
   float zenith[3], horizon[3], ground[3];
@@ -130,6 +135,7 @@ enum Ctr : int {
   kBatch = 0,                            // batch being rendered (advanced on the device)
   kOverflow = 1,                         // lanes whose stack exceeded kMaxStack
   kRecCnt = 2,                           // guiding training records of the batch
+  kWatchdog = 3,                         // waves that hit an iteration bound (see dtrace.h)
   kCnt = 16,                             // [queue][chunk] live paths
   kShCnt = kCnt + 2 * kMaxChunks,        // [chunk] shadow rays
   kCursor = kShCnt + kMaxChunks,         // per-launch work cursors of a batch

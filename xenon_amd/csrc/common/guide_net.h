@@ -11,6 +11,7 @@ constexpr int kGIn = 48;       // 32 grid features + 16 SH coefficients
 constexpr int kGHid = 64;      // hidden width (3 hidden layers)
 constexpr int kGOut = 48;      // 8 lobes x 5 raw values, padded to a multiple of 16
 constexpr int kGLobes = 8;
+constexpr int kMixVecs = 12;   // float4 per stored mixture (guide.h)
 
 // Offsets of the four row-major [in][out] weight matrices.
 constexpr int kW1 = 0, kW2 = kW1 + kGIn * kGHid, kW3 = kW2 + kGHid * kGHid, kW4 = kW3 + kGHid * kGHid,
@@ -18,7 +19,8 @@ constexpr int kW1 = 0, kW2 = kW1 + kGIn * kGHid, kW3 = kW2 + kGHid * kGHid, kW4 
 
 struct GuideNet {
   const uint16_t* grid;   // FP16 bits [level][slot][feature]
-  const uint16_t* w;      // FP16 bits, kWTotal
+  const uint16_t* w;      // FP16 bits, kWTotal, row-major per layer
+  const uint16_t* wt;     // FP16 bits, kWTotal, as MFMA operand tiles (guide.h)
   float lo[3], inv_ext[3];   // position normalisation to the unit cube
   int32_t t_log2;            // table entries per level = 2^t_log2
   int32_t res[kGLevels];     // lattice resolution per level

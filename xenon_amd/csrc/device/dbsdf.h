@@ -228,6 +228,15 @@ __device__ __forceinline__ f3 sss_f(f3 wi, const DMat& m) {
   return lerp3(m.base, m.ss, m.sss) * (1.f - m.metallic) * kInvPiD;
 }
 
+// True when the BSDF is exactly Lambertian: an opaque, non-metal dielectric
+// with F0 = 0 (IOR 1), whose sampler always takes the cosine lobe
+// (bsdf.h:666-708 with a zero specular weight) and whose NEE terms reduce to
+// the diffuse lobe. Path guiding mixes its sampler with this one.
+__device__ __forceinline__ bool lambert_exact(const DMat& m) {
+  return (m.flags & (XNH_MAT_DELTA | XNH_MAT_CONDUCTOR | XNH_MAT_TRANSMISSIVE | XNH_MAT_SUBSURFACE)) == 0u &&
+         m.f0.x == 0.f;
+}
+
 // NEE evaluation summed over non-delta lobes (bsdf.h:601-657).
 __device__ __forceinline__ f3 nee_f(f3 wo, f3 wi, const DMat& m) {
   if (m.delta()) return splat(0.f);

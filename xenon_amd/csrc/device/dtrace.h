@@ -29,8 +29,11 @@
 
 namespace xnh {
 
+// The kernels' dynamic LDS: lds_depth x blockDim.x stack entries.
+extern __shared__ uint32_t trav_lds[];
+
 struct TravStack {
-  uint32_t* lds;        // this lane's column: lds[e * lds_stride]
+  int lds_col;          // this lane's column: trav_lds[e * lds_stride + lds_col]
   uint32_t* spill;      // this lane's spill column: spill[(e - lds_depth) * spill_stride]
   int lds_stride, spill_stride, lds_depth, sp;
   bool overflow;
@@ -39,13 +42,13 @@ struct TravStack {
       overflow = true;
       return;
     }
-    if (sp < lds_depth) lds[sp * lds_stride] = v;
+    if (sp < lds_depth) trav_lds[sp * lds_stride + lds_col] = v;
     else spill[(sp - lds_depth) * spill_stride] = v;
     ++sp;
   }
   __device__ __forceinline__ uint32_t pop() {
     --sp;
-    return (sp < lds_depth) ? lds[sp * lds_stride] : spill[(sp - lds_depth) * spill_stride];
+    return (sp < lds_depth) ? trav_lds[sp * lds_stride + lds_col] : spill[(sp - lds_depth) * spill_stride];
   }
 };
 
@@ -124,18 +127,37 @@ __device__ __forceinline__ InstRec load_inst(const float4* inst, int i) {
   return r;
 }
 
-// Scene walk as a resumable state machine: one step pops one stack entry
-// (a TLAS node, an instance, or a BLAS node pair / leaf), so a persistent
-// kernel can retire a finished ray and start the next one in the same lane
-// while the rest of the wave keeps walking. The visiting order, and hence
-// every result and work counter, equals a recursive TLAS walk with nested
-// BLAS walks: a TLAS leaf pushes its instances (tagged with kInstTag) in
-// reverse so they are walked first to last; an instance's BLAS entries sit
-// above `blas_base` on the shared stack. Closest hit starts from
-// t_best = +inf and ignores tmax, as Scene::intersect does with a default
-// HitRecord; any hit uses [tmin, tmax]. A stack overflow ends the ray with
-// what it has found so far (counted by the caller).
+// Scene walk as a resumable state machine, advanced one round at a time so
+// a persistent kernel can retire a finished ray and start the next in the
+// same lane while the rest of the wave keeps walking.
+//
+// A round runs the lane's TLAS steps until it enters a BLAS walk, then two
+// wave-wide phases over the lanes inside BLAS walks (Aila and Laine's
+// while-while with speculative traversal): an interior phase that walks
+// node pairs, continuing with the nearer hit child in registers and
+// pushing only the farther, and a leaf phase that tests triangles. A lane
+// that reaches a leaf postpones it (one slot) and keeps walking interior
+// nodes until every lane holds a leaf, so both phases run with most lanes
+// busy.
+//
+// Results equal the reference's recursive walk, ties included: triangles
+// are still tested in depth-first order (a postponed leaf is tested before
+// any later leaf), and interior nodes tested before a postponed leaf see a
+// t_best that is too large only by that leaf's hits, which can add visits
+// (counted) but never an accepted triangle (a subtree the reference culls
+// has t > its t_best at that point, which is >= ours when the subtree's
+// triangles are tested). The visiting order otherwise follows the
+// reference: BLAS children nearer entry first, ties right first (closest) /
+// left first (any); TLAS test-on-pop with centre-ordered children
+// (tlas.cpp:231-241); a TLAS leaf's instances (pushed with kInstTag, in
+// reverse) walked first to last. Closest hit starts from t_best = +inf and
+// ignores tmax, as Scene::intersect does with a default HitRecord; any hit
+// uses [tmin, tmax]. A stack overflow ends the ray with what it has found.
 constexpr uint32_t kInstTag = 0x80000000u;
+constexpr uint32_t kEntEnd = kNoRoot;   // no entry left in the BLAS walk
+
+__device__ __forceinline__ bool ent_interior(uint32_t e) { return e != kEntEnd && (e >> kEntShift) == 0u; }
+__device__ __forceinline__ bool ent_leaf(uint32_t e) { return e != kEntEnd && (e >> kEntShift) != 0u; }
 
 template <bool kAny>
 struct Walker {
@@ -144,7 +166,10 @@ struct Walker {
   float tmin, tbest;
   int blas_base;      // stack depth under the current BLAS walk; -1 in the TLAS
   int inst;           // instance of the BLAS walk
+  uint32_t e;         // next BLAS entry to process, or kEntEnd
+  uint32_t leaf;      // postponed leaf entry, 0 = none
   bool inst_found, found;
+  bool stuck;         // an iteration bound was reached (the walk was abandoned)
   HitRef hit;
 
   __device__ __forceinline__ void begin(const DevScene& S, f3 o_, f3 d_, float tmin_, float tmax_, TravStack& st) {
@@ -155,8 +180,11 @@ struct Walker {
     tbest = kAny ? tmax_ : __builtin_huge_valf();
     blas_base = -1;
     inst = -1;
+    e = kEntEnd;
+    leaf = 0u;
     inst_found = false;
     found = false;
+    stuck = false;
     hit = HitRef{0.f, 0.f, 0.f, 0u, -1};
     st.sp = 0;
     st.overflow = false;
@@ -171,133 +199,186 @@ struct Walker {
     blas_base = -1;
   }
 
-  // Advances by one stack entry; returns true once the ray is finished
-  // (closest: `found` / `hit`; any: `found` = occluded).
-  __device__ __forceinline__ bool step(const DevScene& S, TravStack& st, TravCount& c) {
-    if (blas_base >= 0 && st.sp == blas_base) end_instance();
-    if (blas_base >= 0) {
-      const uint32_t e = st.pop();
-      uint32_t count = e >> kEntShift, child = e & kEntIndexMask;
-      if (count == kEntBigLeaf) {
-        const float4 nb = S.bnodes[2 * child + 1];
-        child = __float_as_uint(nb.z);
-        count = __float_as_uint(nb.w);
+  __device__ __forceinline__ uint32_t pop_blas(TravStack& st) { return st.sp > blas_base ? st.pop() : kEntEnd; }
+
+  // TLAS steps until this lane is inside a BLAS walk; true = ray finished.
+  __device__ __forceinline__ bool tlas_steps(const DevScene& S, TravStack& st, TravCount& c) {
+    for (uint32_t guard = 0; blas_base < 0; ++guard) {
+      if (guard >= kPhaseStepLimit) {
+        stuck = true;
+        return true;
       }
-      if (count > 0) {
-        for (uint32_t i = 0; i < count; ++i) {
-          float t, u, v;
-          ++c.tri_tests;
-          if (tri_test(S.tris, child + i, lo, ld, tmin, tbest, t, u, v)) {
-            if (kAny) {
-              found = true;
-              return true;
-            }
-            tbest = t;
-            hit.t = t;
-            hit.u = u;
-            hit.v = v;
-            hit.slot = child + i;
-            inst_found = true;
-          }
-        }
+      if (st.sp == 0) return true;
+      const uint32_t te = st.pop();
+      if (te & kInstTag) {
+        const int ii = static_cast<int>(te & ~kInstTag);
+        const InstRec in = load_inst(S.inst, ii);
+        ++c.inst_visits;
+        lo = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
+        ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
+        linv = f3{1.f / ld.x, 1.f / ld.y, 1.f / ld.z};
+        inst = ii;
+        inst_found = false;
+        blas_base = st.sp;
+        e = in.root;   // kNoRoot (an empty mesh) is kEntEnd
+        leaf = 0u;
         return false;
       }
-      const float4 la = S.bnodes[2 * child], lb = S.bnodes[2 * child + 1];
-      const float4 ra = S.bnodes[2 * child + 2], rb = S.bnodes[2 * child + 3];
-      float tn0, tn1;
-      ++c.interior;
-      c.node_tests += 2;
-      const int m = pair_test(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1);
-      const uint32_t el = blas_entry(child, __float_as_uint(lb.z), __float_as_uint(lb.w));
-      const uint32_t er = blas_entry(child + 1, __float_as_uint(rb.z), __float_as_uint(rb.w));
-      if (kAny) {
-        if (m & 2) st.push(er);
-        if (m & 1) st.push(el);
-      } else if (m == 3) {
-        if (tn0 < tn1) {
-          st.push(er);
-          st.push(el);
-        } else {
-          st.push(el);
-          st.push(er);
-        }
-      } else if (m) {
-        st.push((m & 1) ? el : er);
-      }
-      if (st.overflow) {
-        end_instance();
-        return true;
-      }
-      return false;
-    }
-    if (st.sp == 0) return true;
-    const uint32_t e = st.pop();
-    if (e & kInstTag) {
-      const int ii = static_cast<int>(e & ~kInstTag);
-      const InstRec in = load_inst(S.inst, ii);
-      ++c.inst_visits;
-      lo = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
-      ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
-      linv = f3{1.f / ld.x, 1.f / ld.y, 1.f / ld.z};
-      inst = ii;
-      inst_found = false;
-      blas_base = st.sp;
-      if (in.root != kNoRoot) st.push(in.root);
-      if (st.overflow) {
-        end_instance();
-        return true;
-      }
-      return false;
-    }
-    const uint32_t ni = e;
-    const float4 na = S.tnodes[3 * ni], nb = S.tnodes[3 * ni + 1];
-    ++c.node_tests;
-    // sign-selected slab (geometry/aabb.h:59-71): near/far by direction sign
-    const bool sx = d.x < 0.f, sy = d.y < 0.f, sz = d.z < 0.f;
-    const float bx0 = sx ? na.w : na.x, bx1 = sx ? na.x : na.w;
-    const float by0 = sy ? nb.x : na.y, by1 = sy ? na.y : nb.x;
-    const float bz0 = sz ? nb.y : na.z, bz1 = sz ? na.z : nb.y;
-    const float txn = (bx0 - o.x) * inv.x, txf = (bx1 - o.x) * inv.x;
-    const float tyn = (by0 - o.y) * inv.y, tyf = (by1 - o.y) * inv.y;
-    const float tzn = (bz0 - o.z) * inv.z, tzf = (bz1 - o.z) * inv.z;
-    const float lo_t = smax(smax(smax(tmin, txn), tyn), tzn);
-    const float hi_t = smin(smin(smin(tbest, txf), tyf), tzf);
-    if (!(lo_t <= hi_t)) return false;
-    const uint32_t child = __float_as_uint(nb.z), count = __float_as_uint(nb.w);
-    if (count > 0) {
-      for (uint32_t k = count; k-- > 0;) st.push(kInstTag | S.torder[child + k]);
-    } else {
-      ++c.interior;
-      const float4 nc = S.tnodes[3 * ni + 2];
-      const int ax = static_cast<int>(__float_as_uint(nc.z));
-      const float oa = comp(o, ax), ia = comp(inv, ax);
-      const float tl = (nc.x - oa) * ia;
-      const float tr = (nc.y - oa) * ia;
-      if (kAny) {
-        st.push(child);
-        st.push(child + 1);
-      } else if (tl < tr) {
-        st.push(child + 1);
-        st.push(child);
+      const uint32_t ni = te;
+      const float4 na = S.tnodes[3 * ni], nb = S.tnodes[3 * ni + 1];
+      ++c.node_tests;
+      // sign-selected slab (geometry/aabb.h:59-71): near/far by direction sign
+      const bool sx = d.x < 0.f, sy = d.y < 0.f, sz = d.z < 0.f;
+      const float bx0 = sx ? na.w : na.x, bx1 = sx ? na.x : na.w;
+      const float by0 = sy ? nb.x : na.y, by1 = sy ? na.y : nb.x;
+      const float bz0 = sz ? nb.y : na.z, bz1 = sz ? na.z : nb.y;
+      const float txn = (bx0 - o.x) * inv.x, txf = (bx1 - o.x) * inv.x;
+      const float tyn = (by0 - o.y) * inv.y, tyf = (by1 - o.y) * inv.y;
+      const float tzn = (bz0 - o.z) * inv.z, tzf = (bz1 - o.z) * inv.z;
+      const float lo_t = smax(smax(smax(tmin, txn), tyn), tzn);
+      const float hi_t = smin(smin(smin(tbest, txf), tyf), tzf);
+      if (!(lo_t <= hi_t)) continue;
+      const uint32_t child = __float_as_uint(nb.z), count = __float_as_uint(nb.w);
+      if (count > 0) {
+        for (uint32_t k = count; k-- > 0;) st.push(kInstTag | S.torder[child + k]);
       } else {
-        st.push(child);
-        st.push(child + 1);
+        ++c.interior;
+        const float4 nc = S.tnodes[3 * ni + 2];
+        const int ax = static_cast<int>(__float_as_uint(nc.z));
+        const float oa = comp(o, ax), ia = comp(inv, ax);
+        const float tl = (nc.x - oa) * ia;
+        const float tr = (nc.y - oa) * ia;
+        if (kAny) {
+          st.push(child);
+          st.push(child + 1);
+        } else if (tl < tr) {
+          st.push(child + 1);
+          st.push(child);
+        } else {
+          st.push(child);
+          st.push(child + 1);
+        }
+      }
+      if (st.overflow) return true;
+    }
+    return false;
+  }
+
+  // One interior step on e (an interior entry): pair-test its children,
+  // continue with the nearer hit one, push the farther; a leaf reached
+  // while no leaf is postponed is postponed.
+  __device__ __forceinline__ void interior_step(const DevScene& S, TravStack& st, TravCount& c) {
+    const uint32_t child = e;
+    const float4 la = S.bnodes[2 * child], lb = S.bnodes[2 * child + 1];
+    const float4 ra = S.bnodes[2 * child + 2], rb = S.bnodes[2 * child + 3];
+    float tn0, tn1;
+    ++c.interior;
+    c.node_tests += 2;
+    const int m = pair_test(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1);
+    const uint32_t el = blas_entry(child, __float_as_uint(lb.z), __float_as_uint(lb.w));
+    const uint32_t er = blas_entry(child + 1, __float_as_uint(rb.z), __float_as_uint(rb.w));
+    uint32_t next;
+    if (m == 3) {
+      // closest: nearer first, ties right first; any: left first
+      const bool left_first = kAny || tn0 < tn1;
+      st.push(left_first ? er : el);
+      next = left_first ? el : er;
+    } else if (m) {
+      next = (m & 1) ? el : er;
+    } else {
+      next = pop_blas(st);
+    }
+    if (leaf == 0u && ent_leaf(next)) {
+      leaf = next;
+      next = pop_blas(st);
+    }
+    e = next;
+  }
+
+  // Tests the postponed leaf; true when an any-hit walk found an occluder.
+  __device__ __forceinline__ bool leaf_step(const DevScene& S, TravStack& st, TravCount& c) {
+    uint32_t count = leaf >> kEntShift, first = leaf & kEntIndexMask;
+    if (count == kEntBigLeaf) {
+      const float4 nb = S.bnodes[2 * first + 1];
+      first = __float_as_uint(nb.z);
+      count = __float_as_uint(nb.w);
+    }
+    for (uint32_t k = 0; k < count; ++k) {
+      float t, u, v;
+      ++c.tri_tests;
+      if (tri_test(S.tris, first + k, lo, ld, tmin, tbest, t, u, v)) {
+        if (kAny) {
+          found = true;
+          return true;
+        }
+        tbest = t;
+        hit.t = t;
+        hit.u = u;
+        hit.v = v;
+        hit.slot = first + k;
+        inst_found = true;
       }
     }
-    return st.overflow;
+    leaf = 0u;
+    if (ent_leaf(e)) {
+      leaf = e;
+      e = pop_blas(st);
+    }
+    return false;
+  }
+
+  // One round; true once the ray is finished (closest: `found` / `hit`;
+  // any: `found` = occluded). Reached by the lanes that hold a ray; the
+  // phases are wave-wide over those lanes.
+  __device__ __forceinline__ bool round(const DevScene& S, TravStack& st, TravCount& c) {
+    if (tlas_steps(S, st, c)) return true;
+    for (uint32_t guard = 0;; ++guard) {
+      const bool walk = ent_interior(e);
+      // stop when no lane can walk or every walking lane holds a leaf
+      if (__ballot(walk && leaf == 0u) == 0ull) break;
+      if (guard >= kPhaseStepLimit) {
+        stuck = true;
+        return true;
+      }
+      if (walk) interior_step(S, st, c);
+      if (st.overflow) break;
+    }
+    if (st.overflow) {
+      end_instance();
+      return true;
+    }
+    for (uint32_t guard = 0;; ++guard) {
+      const bool has = leaf != 0u;
+      if (__ballot(has) == 0ull) break;
+      if (guard >= kPhaseStepLimit) {
+        stuck = true;
+        return true;
+      }
+      if (has && leaf_step(S, st, c)) return true;
+    }
+    if (e == kEntEnd && leaf == 0u) end_instance();
+    return false;
   }
 };
 
-// Whole walk of one ray (the ray-batch API).
+// Whole walk of one ray per lane (the ray-batch API). Every lane of the
+// wave must reach it.
 template <bool kAny>
-__device__ __forceinline__ bool walk_scene(const DevScene& S, f3 o, f3 d, float tmin, float tmax, HitRef& hit,
-                                           TravStack& st, TravCount& c) {
+__device__ __forceinline__ bool walk_scene(const DevScene& S, f3 o, f3 d, float tmin, float tmax, bool valid,
+                                           HitRef& hit, TravStack& st, TravCount& c) {
   Walker<kAny> w;
   w.begin(S, o, d, tmin, tmax, st);
-  while (!w.step(S, st, c)) {
+  bool done = !valid;
+  for (uint32_t guard = 0; __ballot(!done) != 0ull; ++guard) {
+    if (guard >= kWaveRoundLimit) {
+      w.stuck = true;
+      break;
+    }
+    if (!done) done = w.round(S, st, c);
   }
+  if (w.stuck) st.overflow = true;
   hit = w.hit;
-  return w.found;
+  return valid && w.found;
 }
 
 }  // namespace xnh

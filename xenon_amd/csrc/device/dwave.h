@@ -1,0 +1,169 @@
+// dwave.h — wave-level helpers and batch bookkeeping shared by the
+// wavefront kernels (kernels.hip) and the guided-rendering kernels
+// (guide.hip): ballot compaction, wave reductions, traversal-counter
+// flushes, the per-block view of the current batch's chunk table, and the
+// surface record of a closest hit.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../common/dev_types.h"
+#include "dmath.h"
+#include "dtrace.h"
+
+namespace xnh {
+
+constexpr float kBounceOffset = 1e-4f;   // math/ray.h:13
+constexpr float kShadowOff = 5e-4f;      // math/ray.h:14
+
+__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
+
+// Wave-aggregated append: one atomic per wave, ranks by ballot prefix.
+// Must be reached by every lane of the wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {
+  const unsigned long long mask = __ballot(want);
+  if (mask == 0ull) return 0u;
+  const int lane = lane_id();
+  const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(ctr, static_cast<uint32_t>(__popcll(mask)));
+  base = __shfl(base, leader);
+  return base + rank;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Wave-reduced traversal counters; `any` selects the shadow slots.
+__device__ __forceinline__ void flush_counts(unsigned long long* stats, const TravCount& c, uint32_t rays, bool any) {
+  const unsigned long long a = wave_sum(c.node_tests), b = wave_sum(c.interior), t = wave_sum(c.tri_tests),
+                           i = wave_sum(c.inst_visits);
+  const unsigned long long r = wave_sum(rays);
+  const int base = any ? kStShNodeTests : kStNodeTests;
+  if (lane_id() == 0) {
+    if (a) atomicAdd(&stats[base], a);
+    if (b) atomicAdd(&stats[base + 1], b);
+    if (t) atomicAdd(&stats[base + 2], t);
+    if (i) atomicAdd(&stats[base + 3], i);
+    if (r) atomicAdd(&stats[any ? kStShadow : kStClosest], r);
+  }
+}
+
+__device__ __forceinline__ TravStack make_stack(uint32_t* spill, int lds_depth) {
+  TravStack st;
+  st.lds_col = static_cast<int>(threadIdx.x);
+  st.lds_stride = static_cast<int>(blockDim.x);
+  st.spill = spill + (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+  st.spill_stride = static_cast<int>(gridDim.x * blockDim.x);
+  st.lds_depth = lds_depth;
+  st.sp = 0;
+  st.overflow = false;
+  return st;
+}
+
+// Surface data of a closest hit: the reference's HitRecord after
+// intersect_triangle (geometry/mesh.h:104-129) and TLAS::fixup_hit
+// (geometry/tlas.cpp:156-168).
+struct Surface {
+  f3 pos, n, gn;
+  int mat;
+  uint32_t tri;
+  int32_t inst_id;
+  int32_t lmap;
+  bool front;
+};
+
+__device__ __forceinline__ f3 normal_out(const InstRec& in, f3 n) {
+  return normalize(f3{in.c0.x * n.x + in.c0.y * n.y + in.c0.z * n.z, in.c1.x * n.x + in.c1.y * n.y + in.c1.z * n.z,
+                      in.c2.x * n.x + in.c2.y * n.y + in.c2.z * n.z});
+}
+
+__device__ __forceinline__ Surface resolve(const DevScene& S, const HitRef& h, f3 o, f3 d) {
+  const InstRec in = load_inst(S.inst, h.inst);
+  const float4 a = S.tris[3 * h.slot], b = S.tris[3 * h.slot + 1], c = S.tris[3 * h.slot + 2];
+  const f3 e1{a.w, b.x, b.y}, e2{b.z, b.w, c.x};
+  Surface s;
+  s.tri = __float_as_uint(c.y);
+  const uint32_t gtri = (in.tri_base & ~kFlatBit) + s.tri;
+  const f3 gl = normalize(cross(e1, e2));
+  f3 nl = gl;
+  if (!(in.tri_base & kFlatBit)) {
+    const int i0 = S.vidx[3 * gtri], i1 = S.vidx[3 * gtri + 1], i2 = S.vidx[3 * gtri + 2];
+    const f3 n0 = xyz(S.vnrm[i0]), n1 = xyz(S.vnrm[i1]), n2 = xyz(S.vnrm[i2]);
+    nl = normalize(n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v);
+  }
+  s.pos = o + h.t * d;
+  s.n = normal_out(in, nl);
+  s.gn = normal_out(in, gl);
+  s.front = dot(s.gn, d) < 0.f;
+  s.mat = S.tmat[gtri];
+  s.inst_id = in.id;
+  s.lmap = in.lmap;
+  return s;
+}
+
+// Gradient sky (scene/environment.h:34-46).
+__device__ __forceinline__ f3 sky_eval(const DevSky& k, f3 d) {
+  const f3 hz{k.horizon[0], k.horizon[1], k.horizon[2]};
+  if (d.y < 0.f) {
+    const float t = smin(-d.y, 1.f);
+    return k.intensity * lerp3(hz, f3{k.ground[0], k.ground[1], k.ground[2]}, t);
+  }
+  const float t = powf(d.y, 1.0f / k.sharpness);
+  return k.intensity * lerp3(hz, f3{k.zenith[0], k.zenith[1], k.zenith[2]}, t);
+}
+
+
+// ---- batch bookkeeping ------------------------------------------------------
+
+// Per-block copy of the current batch's chunk table with prefix sums of
+// 64-entry work units (traversal / shading) or of pixels (accumulation).
+struct BatchView {
+  int batch, nch;
+  DevChunk ch[kMaxChunks];
+  uint32_t cnt[kMaxChunks];
+  uint32_t prefix[kMaxChunks + 1];
+};
+
+// Loads the batch and builds prefix[] from per-chunk counts (counts may be
+// null: prefix over pixels). Block-wide; ends with a barrier.
+__device__ __forceinline__ void load_batch(BatchView& v, const WaveBuffers& B, const uint32_t* counts,
+                                           bool pixels) {
+  if (threadIdx.x == 0) {
+    v.batch = static_cast<int>(B.counters[kBatch]);
+    v.nch = B.nchunks[v.batch];
+  }
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < v.nch) {
+    v.ch[threadIdx.x] = B.chunks[v.batch * kMaxChunks + threadIdx.x];
+    v.cnt[threadIdx.x] = counts ? counts[threadIdx.x] : 0u;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    v.prefix[0] = 0;
+    for (int c = 0; c < v.nch; ++c) {
+      acc += pixels ? static_cast<uint32_t>(v.ch[c].npix) : ((v.cnt[c] + 63u) >> 6);
+      v.prefix[c + 1] = acc;
+    }
+  }
+  __syncthreads();
+}
+
+// Last chunk c with prefix[c] <= u: the (non-empty) chunk holding item u.
+__device__ __forceinline__ int find_chunk(const BatchView& v, uint32_t u) {
+  int lo = 0, hi = v.nch;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (v.prefix[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+
+}  // namespace xnh

@@ -156,6 +156,59 @@ __device__ __forceinline__ void dense_fwd(const _Float16* X, int K, const _Float
   __syncthreads();
 }
 
+// ---- register-resident forward pass -----------------------------------------------
+//
+// The MLP evaluated transposed, Y^T = W^T X^T: A = a 16x16 tile of W^T, B =
+// 16 features x 16 samples of X^T. The accumulator of one layer (lane l:
+// feature 16 mt + 4 (l/16) + j of sample l % 16) is then exactly the B
+// operand of the next layer, so hidden activations never leave registers.
+// The A tiles are stored pre-arranged (GuideNet::wt): layer L (K x N) holds
+// tile (mt, kt) at L's offset + (mt * K/16 + kt) * 256, lane l's four values
+// W[16 kt + 4 (l/16) + j][16 mt + l % 16] at + 4 l + j.
+__device__ __forceinline__ int tiled_index(int k, int n, int K) {
+  return ((n >> 4) * (K >> 4) + (k >> 4)) * 256 + ((((k & 15) >> 2) << 4) + (n & 15)) * 4 + (k & 3);
+}
+
+template <int KT, int MT>
+__device__ __forceinline__ void dense_reg(const h4v* x, const h4v* w, h4v* y) {
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  for (int mt = 0; mt < MT; ++mt) {
+    f4v acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kt = 0; kt < KT; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(w[(mt * KT + kt) * 64 + lane], x[kt], acc, 0, 0, 0);
+    h4v r;
+    for (int j = 0; j < 4; ++j) r[j] = static_cast<_Float16>(fmaxf(acc[j], 0.f));
+    y[mt] = r;
+  }
+}
+
+// wave: forward pass of 64 points, one per lane (p: world position, n:
+// normal). lds: 64 x kGIn FP16 scratch of this wave. On return the lane's
+// 48 raw outputs (FP32 accumulation, stored FP16) are in lds[lane * kGIn ..].
+__device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3 n, _Float16* lds) {
+  const int lane = static_cast<int>(threadIdx.x & 63u), g = lane >> 4, li = lane & 15;
+  guide_encode(net, p, n, lds + lane * kGIn);
+  __syncthreads();
+  h4v x[4][3];
+  for (int nt = 0; nt < 4; ++nt)
+    for (int kt = 0; kt < 3; ++kt)
+      x[nt][kt] = *reinterpret_cast<const h4v*>(lds + (16 * nt + li) * kGIn + 16 * kt + 4 * g);
+  __syncthreads();
+  const h4v* wt = reinterpret_cast<const h4v*>(net.wt);
+  for (int nt = 0; nt < 4; ++nt) {
+    h4v a[4], b[4];
+    dense_reg<3, 4>(x[nt], wt + kW1 / 4, a);
+    dense_reg<4, 4>(a, wt + kW2 / 4, b);
+    dense_reg<4, 4>(b, wt + kW3 / 4, a);
+    for (int mt = 0; mt < kGOut / 16; ++mt) {
+      f4v acc = {0.f, 0.f, 0.f, 0.f};
+      for (int kt = 0; kt < 4; ++kt)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wt[kW4 / 4 + (mt * 4 + kt) * 64 + lane], a[kt], acc, 0, 0, 0);
+      for (int j = 0; j < 4; ++j) lds[(16 * nt + li) * kGIn + 16 * mt + 4 * g + j] = static_cast<_Float16>(acc[j]);
+    }
+  }
+  __syncthreads();
+}
+
 // wave: the full forward pass; lds_x holds the encodings (64 x 48).
 __device__ __forceinline__ void mlp_forward(const GuideNet& net, const _Float16* X, _Float16* H1, _Float16* H2,
                                             _Float16* H3, float* out) {
@@ -213,7 +266,7 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u
 //   k = 0..7   {mu_j.xyz, kappa_j}
 //   k = 8, 9   {pi_0..3}, {pi_4..7}
 //   k = 10, 11 pi_j * C(kappa_j), the pdf weight of lobe j
-constexpr int kMixVecs = 12;
+// (kMixVecs = 12, guide_net.h)
 
 __device__ __forceinline__ void store_mixture(float4* mix, uint32_t C, uint32_t i, const Mixture& m) {
   float w[kGLobes];
@@ -268,5 +321,15 @@ __device__ __forceinline__ f3 stored_sample(const float4* mix, uint32_t C, uint3
 // radiance sum when its continuation reached the next vertex (written
 // there), so the radiance gathered after the vertex is total - prefix.
 constexpr uint32_t kNoRecord = 0xFFFFFFFFu;
+
+// Whether the path in `slot` records training samples in batch `batch`: a
+// hashed 1-in-`every` choice that changes from batch to batch.
+__device__ __forceinline__ bool record_path(uint32_t slot, int batch, uint32_t every) {
+  uint32_t h = slot * 0x9E3779B1u ^ (static_cast<uint32_t>(batch) * 0x85EBCA77u + 0x165667B1u);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h % every == 0u;
+}
 
 }  // namespace xnh

@@ -16,6 +16,8 @@
 // over the batch (FP32 atomics); the caller divides by the batch size in Adam.
 #include <hip/hip_runtime.h>
 
+#include "dbsdf.h"
+#include "dwave.h"
 #include "guide.h"
 #include "launch.h"
 
@@ -113,17 +115,15 @@ struct TrainLds {
 // Evaluates the network for n points: raw outputs (n x 48, FP32).
 __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const float* pos, const float* nrm,
                                                    float* raw) {
-  __shared__ TrainLds s;
+  __shared__ _Float16 lds[64 * kGIn];
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
   const bool ok = i < n;
   const f3 p = ok ? f3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]} : f3{net.lo[0], net.lo[1], net.lo[2]};
   const f3 nn = ok ? f3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]} : f3{0.f, 0.f, 1.f};
-  guide_encode(net, p, nn, s.X + lane * kGIn);
-  __syncthreads();
-  mlp_forward(net, s.X, s.H1, s.H2, s.H3, s.F);
+  guide_forward_wave(net, p, nn, lds);
   if (ok)
-    for (int k = 0; k < kGOut; ++k) raw[static_cast<size_t>(i) * kGOut + k] = s.F[lane * kGOut + k];
+    for (int k = 0; k < kGOut; ++k) raw[static_cast<size_t>(i) * kGOut + k] = static_cast<float>(lds[lane * kGIn + k]);
 }
 
 // Mixture queries for n points: with u non-null, draws one direction per
@@ -131,18 +131,18 @@ __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const fl
 // dir; then writes the mixture pdf at dir[i] (drawn or given) to pdf.
 __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const float* pos, const float* nrm,
                                                     const float* u, float* dir, float* pdf) {
-  __shared__ TrainLds s;
+  __shared__ _Float16 lds[64 * kGIn];
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
   const bool ok = i < n;
   const f3 p = ok ? f3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]} : f3{net.lo[0], net.lo[1], net.lo[2]};
   const f3 nn = ok ? f3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]} : f3{0.f, 0.f, 1.f};
-  guide_encode(net, p, nn, s.X + lane * kGIn);
-  __syncthreads();
-  mlp_forward(net, s.X, s.H1, s.H2, s.H3, s.F);
+  guide_forward_wave(net, p, nn, lds);
   if (!ok) return;
+  float raw[kGOut];
+  for (int k = 0; k < kGLobes * 5; ++k) raw[k] = static_cast<float>(lds[lane * kGIn + k]);
   Mixture m;
-  decode_mixture(s.F + lane * kGOut, m);
+  decode_mixture(raw, m);
   f3 w;
   if (u) {
     w = mixture_sample(m, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
@@ -155,13 +155,130 @@ __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const f
   pdf[i] = mixture_pdf(m, w);
 }
 
+// ---------------------------------------------------------------------------
+// Guided rendering
+// ---------------------------------------------------------------------------
+
+// Evaluates the guide at every closest hit of queue qi that lands on a
+// guided surface (an exact Lambertian that is not an emitter) and stores
+// the mixture for k_shade. One wave per block, grid-stride over the
+// batch's 64-entry units.
+__global__ void __launch_bounds__(64) k_guide_infer(WaveBuffers B, int qi, GuideNet net) {
+  __shared__ BatchView v;
+  __shared__ _Float16 lds[64 * kGIn];
+  const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
+  load_batch(v, B, counts, false);
+  const uint32_t total = v.prefix[v.nch];
+  const PathQueue q = B.q[qi];
+  const uint32_t C = static_cast<uint32_t>(B.capacity);
+  for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
+    const int ci = uniform(find_chunk(v, unit));
+    const DevChunk& ck = v.ch[ci];
+    const DevScene& S = B.jobs[uniform(ck.job)].S;
+    const uint32_t off = (unit - v.prefix[ci]) * 64u + static_cast<uint32_t>(glane());
+    const uint32_t i = static_cast<uint32_t>(ck.base) + off;
+    bool want = false;
+    f3 p{net.lo[0], net.lo[1], net.lo[2]}, n{0.f, 0.f, 1.f};
+    if (off < v.cnt[ci]) {
+      const float4 hv = B.hit[i];
+      const int inst = __float_as_int(hv.w);
+      if (inst >= 0) {
+        const float4 r0 = q.ray0[i], r1 = q.ray1[i];
+        const HitRef h{hv.x, hv.y, hv.z, B.hit_slot[i], inst};
+        const Surface sf = resolve(S, h, f3{r0.x, r0.y, r0.z}, f3{r0.w, r1.x, r1.y});
+        const bool emitter = sf.lmap >= 0 && S.lmap[sf.lmap + static_cast<int>(sf.tri)] >= 0;
+        if (!emitter && lambert_exact(load_mat(S.mats, sf.mat))) {
+          want = true;
+          p = sf.pos;
+          n = sf.n;
+        }
+      }
+    }
+    if (__ballot(want) == 0ull) continue;
+    guide_forward_wave(net, p, n, lds);
+    if (want) {
+      float raw[kGOut];
+      for (int k = 0; k < kGLobes * 5; ++k) raw[k] = static_cast<float>(lds[glane() * kGIn + k]);
+      Mixture m;
+      decode_mixture(raw, m);
+      store_mixture(B.mix, C, i, m);
+    }
+  }
+}
+
+// After a guided batch: turns its training records into training arrays
+// (position, normal, direction, lambda), adds the lambdas to *lam_sum and
+// writes the record count to *n_out.
+// lambda = luminance((L - prefix) / T): the radiance the path gathered after
+// the vertex over the throughput that reached it, i.e. f cos / p times the
+// incident radiance along the sampled direction (the weight of the KL
+// objective of Mueller et al. 2019). Records whose prefix was never written
+// get lambda 0.
+__global__ void __launch_bounds__(256) k_guide_collect(WaveBuffers B, float* pos, float* nrm, float* dir, float* lam,
+                                                       float* lam_sum, uint32_t* n_out) {
+  const uint32_t n = min(B.counters[kRecCnt], B.rec_cap);
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) *n_out = n;
+  float l = 0.f;
+  if (r < n) {
+    const uint32_t R = B.rec_cap;
+    const float4 a = B.rec[r], b = B.rec[R + r], c = B.rec[2 * R + r], e = B.rec[3 * R + r];
+    const uint32_t slot = __float_as_uint(e.w);
+    const float4 t = B.l_term[slot], n0 = B.l_nee[2 * slot], n1 = B.l_nee[2 * slot + 1];
+    const float ar = t.x + n0.x + n1.x - e.x, ag = t.y + n0.y + n1.y - e.y, ab = t.z + n0.z + n1.z - e.z;
+    const float lr = a.w > 0.f ? ar / a.w : 0.f, lg = b.w > 0.f ? ag / b.w : 0.f, lb = c.w > 0.f ? ab / c.w : 0.f;
+    l = 0.2126f * lr + 0.7152f * lg + 0.0722f * lb;
+    if (!(l > 0.f) || !(l < 3.0e38f)) l = 0.f;
+    pos[3 * r] = a.x;
+    pos[3 * r + 1] = a.y;
+    pos[3 * r + 2] = a.z;
+    nrm[3 * r] = b.x;
+    nrm[3 * r + 1] = b.y;
+    nrm[3 * r + 2] = b.z;
+    dir[3 * r] = c.x;
+    dir[3 * r + 1] = c.y;
+    dir[3 * r + 2] = c.z;
+    lam[r] = l;
+  }
+  const float ws = g_wave_sumf(l);
+  if (glane() == 0 && ws > 0.f) atomicAdd(lam_sum, ws);
+}
+
+// Rearranges the FP16 row-major weights into MFMA operand tiles.
+__global__ void __launch_bounds__(256) k_tile_weights(const uint16_t* w, uint16_t* wt) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= kWTotal) return;
+  int off, K, N;
+  if (i < kW2) {
+    off = kW1; K = kGIn; N = kGHid;
+  } else if (i < kW3) {
+    off = kW2; K = kGHid; N = kGHid;
+  } else if (i < kW4) {
+    off = kW3; K = kGHid; N = kGHid;
+  } else {
+    off = kW4; K = kGHid; N = kGOut;
+  }
+  const int r = i - off, k = r / N, c = r - k * N;
+  wt[off + tiled_index(k, c, K)] = w[i];
+}
+
 // Forward + loss + backward over n training samples; accumulates the
 // gradients of sum_i l_i into grid_g (L x T x 2) and w_g (kWTotal), and the
 // summed loss into loss_sum.
+// With n_dev non-null the sample count is min(*n_dev, n) and lam_scale is
+// taken as count / *lam_sum_dev (the batch mean of lambda scaled to 1).
 __global__ void __launch_bounds__(64) k_guide_train(GuideNet net, int n, const float* pos, const float* nrm,
                                                     const float* dir, const float* lam, float lam_scale,
+                                                    const uint32_t* n_dev, const float* lam_sum_dev,
                                                     float* grid_g, float* w_g, float* loss_sum) {
   __shared__ TrainLds s;
+  if (n_dev) {
+    const uint32_t cnt = min(*n_dev, static_cast<uint32_t>(n));
+    const float sum = *lam_sum_dev;
+    n = static_cast<int>(cnt);
+    lam_scale = sum > 0.f ? static_cast<float>(cnt) / sum : 0.f;
+  }
+  if (static_cast<int>(blockIdx.x) * 64 >= n) return;
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
   const bool ok = i < n;
@@ -211,11 +328,17 @@ __global__ void __launch_bounds__(64) k_guide_train(GuideNet net, int n, const f
 
 // Adam with bias correction on FP32 masters; refreshes the FP16 copies and
 // clears the gradients. grad_scale converts summed gradients to means.
+// With n_dev non-null the gradient scale is 1 / *n_dev (no step when zero).
 __global__ void __launch_bounds__(256) k_guide_adam(float* master, uint16_t* half_bits, float* grad, float* m,
                                                     float* v, size_t count, float lr, float b1, float b2, float eps,
-                                                    float bc1, float bc2, float grad_scale) {
+                                                    float bc1, float bc2, float grad_scale, const uint32_t* n_dev) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= count) return;
+  if (n_dev) {
+    const uint32_t cnt = *n_dev;
+    if (cnt == 0) return;
+    grad_scale = 1.f / static_cast<float>(cnt);
+  }
   const float g = grad[i] * grad_scale;
   const float mi = b1 * m[i] + (1.f - b1) * g;
   const float vi = b2 * v[i] + (1.f - b2) * g * g;
@@ -247,20 +370,37 @@ hipError_t launch_guide_query(hipStream_t s, const GuideNet& net, int n, const f
 }
 
 hipError_t launch_guide_train(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
-                              const float* dir, const float* lam, float lam_scale, float* grid_g, float* w_g,
-                              float* loss_sum) {
+                              const float* dir, const float* lam, float lam_scale, const uint32_t* n_dev,
+                              const float* lam_sum_dev, float* grid_g, float* w_g, float* loss_sum) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_guide_train, dim3((n + 63) / 64), dim3(64), 0, s, net, n, pos, nrm, dir, lam, lam_scale,
-                     grid_g, w_g, loss_sum);
+  hipLaunchKernelGGL(k_guide_train, dim3((n + 63) / 64), dim3(64), 0, s, net, n, pos, nrm, dir, lam, lam_scale, n_dev,
+                     lam_sum_dev, grid_g, w_g, loss_sum);
   return hipGetLastError();
 }
 
 hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, float* grad, float* m, float* v,
                              size_t count, float lr, float b1, float b2, float eps, float bc1, float bc2,
-                             float grad_scale) {
+                             float grad_scale, const uint32_t* n_dev) {
   const size_t blocks = (count + 255) / 256;
   hipLaunchKernelGGL(k_guide_adam, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, master, half_copy, grad, m,
-                     v, count, lr, b1, b2, eps, bc1, bc2, grad_scale);
+                     v, count, lr, b1, b2, eps, bc1, bc2, grad_scale, n_dev);
+  return hipGetLastError();
+}
+
+hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, const GuideNet& net, int grid) {
+  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, s, B, qi, net);
+  return hipGetLastError();
+}
+
+hipError_t launch_guide_collect(hipStream_t s, const WaveBuffers& B, float* pos, float* nrm, float* dir, float* lam,
+                                float* lam_sum, uint32_t* n_out) {
+  const uint32_t n = B.rec_cap > 0u ? B.rec_cap : 1u;
+  hipLaunchKernelGGL(k_guide_collect, dim3((n + 255) / 256), dim3(256), 0, s, B, pos, nrm, dir, lam, lam_sum, n_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_weights(hipStream_t s, const uint16_t* w, uint16_t* wt) {
+  hipLaunchKernelGGL(k_tile_weights, dim3((kWTotal + 255) / 256), dim3(256), 0, s, w, wt);
   return hipGetLastError();
 }
 

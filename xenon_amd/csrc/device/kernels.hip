@@ -21,170 +21,11 @@
 #include "dbsdf.h"
 #include "dmath.h"
 #include "dtrace.h"
+#include "dwave.h"
+#include "guide.h"
 #include "launch.h"
 
 namespace xnh {
-
-namespace {
-
-constexpr float kBounceOffset = 1e-4f;   // math/ray.h:13
-constexpr float kShadowOff = 5e-4f;      // math/ray.h:14
-
-__device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
-
-// Wave-aggregated append: one atomic per wave, ranks by ballot prefix.
-// Must be reached by every lane of the wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {
-  const unsigned long long mask = __ballot(want);
-  if (mask == 0ull) return 0u;
-  const int lane = lane_id();
-  const uint32_t rank = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
-  const int leader = __ffsll(static_cast<long long>(mask)) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(ctr, static_cast<uint32_t>(__popcll(mask)));
-  base = __shfl(base, leader);
-  return base + rank;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-// Wave-reduced traversal counters; `any` selects the shadow slots.
-__device__ __forceinline__ void flush_counts(unsigned long long* stats, const TravCount& c, uint32_t rays, bool any) {
-  const unsigned long long a = wave_sum(c.node_tests), b = wave_sum(c.interior), t = wave_sum(c.tri_tests),
-                           i = wave_sum(c.inst_visits);
-  const unsigned long long r = wave_sum(rays);
-  const int base = any ? kStShNodeTests : kStNodeTests;
-  if (lane_id() == 0) {
-    if (a) atomicAdd(&stats[base], a);
-    if (b) atomicAdd(&stats[base + 1], b);
-    if (t) atomicAdd(&stats[base + 2], t);
-    if (i) atomicAdd(&stats[base + 3], i);
-    if (r) atomicAdd(&stats[any ? kStShadow : kStClosest], r);
-  }
-}
-
-__device__ __forceinline__ TravStack make_stack(uint32_t* lds, uint32_t* spill, int lds_depth) {
-  TravStack st;
-  st.lds = lds + threadIdx.x;
-  st.lds_stride = static_cast<int>(blockDim.x);
-  st.spill = spill + (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x);
-  st.spill_stride = static_cast<int>(gridDim.x * blockDim.x);
-  st.lds_depth = lds_depth;
-  st.sp = 0;
-  st.overflow = false;
-  return st;
-}
-
-// Surface data of a closest hit: the reference's HitRecord after
-// intersect_triangle (geometry/mesh.h:104-129) and TLAS::fixup_hit
-// (geometry/tlas.cpp:156-168).
-struct Surface {
-  f3 pos, n, gn;
-  int mat;
-  uint32_t tri;
-  int32_t inst_id;
-  int32_t lmap;
-  bool front;
-};
-
-__device__ __forceinline__ f3 normal_out(const InstRec& in, f3 n) {
-  return normalize(f3{in.c0.x * n.x + in.c0.y * n.y + in.c0.z * n.z, in.c1.x * n.x + in.c1.y * n.y + in.c1.z * n.z,
-                      in.c2.x * n.x + in.c2.y * n.y + in.c2.z * n.z});
-}
-
-__device__ __forceinline__ Surface resolve(const DevScene& S, const HitRef& h, f3 o, f3 d) {
-  const InstRec in = load_inst(S.inst, h.inst);
-  const float4 a = S.tris[3 * h.slot], b = S.tris[3 * h.slot + 1], c = S.tris[3 * h.slot + 2];
-  const f3 e1{a.w, b.x, b.y}, e2{b.z, b.w, c.x};
-  Surface s;
-  s.tri = __float_as_uint(c.y);
-  const uint32_t gtri = (in.tri_base & ~kFlatBit) + s.tri;
-  const f3 gl = normalize(cross(e1, e2));
-  f3 nl = gl;
-  if (!(in.tri_base & kFlatBit)) {
-    const int i0 = S.vidx[3 * gtri], i1 = S.vidx[3 * gtri + 1], i2 = S.vidx[3 * gtri + 2];
-    const f3 n0 = xyz(S.vnrm[i0]), n1 = xyz(S.vnrm[i1]), n2 = xyz(S.vnrm[i2]);
-    nl = normalize(n0 * (1.f - h.u - h.v) + n1 * h.u + n2 * h.v);
-  }
-  s.pos = o + h.t * d;
-  s.n = normal_out(in, nl);
-  s.gn = normal_out(in, gl);
-  s.front = dot(s.gn, d) < 0.f;
-  s.mat = S.tmat[gtri];
-  s.inst_id = in.id;
-  s.lmap = in.lmap;
-  return s;
-}
-
-// Gradient sky (scene/environment.h:34-46).
-__device__ __forceinline__ f3 sky_eval(const DevSky& k, f3 d) {
-  const f3 hz{k.horizon[0], k.horizon[1], k.horizon[2]};
-  if (d.y < 0.f) {
-    const float t = smin(-d.y, 1.f);
-    return k.intensity * lerp3(hz, f3{k.ground[0], k.ground[1], k.ground[2]}, t);
-  }
-  const float t = powf(d.y, 1.0f / k.sharpness);
-  return k.intensity * lerp3(hz, f3{k.zenith[0], k.zenith[1], k.zenith[2]}, t);
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// Batch bookkeeping shared by the wavefront kernels.
-// ---------------------------------------------------------------------------
-namespace {
-
-// Per-block copy of the current batch's chunk table with prefix sums of
-// 64-entry work units (traversal / shading) or of pixels (accumulation).
-struct BatchView {
-  int batch, nch;
-  DevChunk ch[kMaxChunks];
-  uint32_t cnt[kMaxChunks];
-  uint32_t prefix[kMaxChunks + 1];
-};
-
-// Loads the batch and builds prefix[] from per-chunk counts (counts may be
-// null: prefix over pixels). Block-wide; ends with a barrier.
-__device__ __forceinline__ void load_batch(BatchView& v, const WaveBuffers& B, const uint32_t* counts,
-                                           bool pixels) {
-  if (threadIdx.x == 0) {
-    v.batch = static_cast<int>(B.counters[kBatch]);
-    v.nch = B.nchunks[v.batch];
-  }
-  __syncthreads();
-  if (static_cast<int>(threadIdx.x) < v.nch) {
-    v.ch[threadIdx.x] = B.chunks[v.batch * kMaxChunks + threadIdx.x];
-    v.cnt[threadIdx.x] = counts ? counts[threadIdx.x] : 0u;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t acc = 0;
-    v.prefix[0] = 0;
-    for (int c = 0; c < v.nch; ++c) {
-      acc += pixels ? static_cast<uint32_t>(v.ch[c].npix) : ((v.cnt[c] + 63u) >> 6);
-      v.prefix[c + 1] = acc;
-    }
-  }
-  __syncthreads();
-}
-
-// Last chunk c with prefix[c] <= u: the (non-empty) chunk holding item u.
-__device__ __forceinline__ int find_chunk(const BatchView& v, uint32_t u) {
-  int lo = 0, hi = v.nch;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (v.prefix[mid] <= u) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-__device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // raygen: per-(pixel, sample) seed and jittered pinhole ray
@@ -208,6 +49,7 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
     B.counters[kShCnt + gid] = 0u;
   }
   if (gid < kNumCursors) B.counters[kCursor + gid] = 0u;
+  if (gid == 0) B.counters[kRecCnt] = 0u;
   if (v.nch == 0) return;
   const DevChunk& last = v.ch[v.nch - 1];
   if (gid >= last.base + last.cap) return;
@@ -239,7 +81,7 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
   q.ray1[gid] = make_float4(d.y, d.z, 0.f, __uint_as_float(1u));
   q.thr[gid] = make_float4(1.f, 1.f, 1.f, 0.f);
   q.st[gid] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), static_cast<uint32_t>(gid),
-                         0u);
+                         kNoRecord);
   B.l_term[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
   B.l_nee[2 * gid] = make_float4(0.f, 0.f, 0.f, 0.f);
   B.l_nee[2 * gid + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -269,7 +111,12 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
   int pci = 0;
   int held = -1;                  // fetched unit of another job, waiting for the wave to drain
   bool exhausted = false;
-  for (;;) {
+  bool stuck = false;
+  for (uint32_t guard = 0;; ++guard) {
+    if (guard >= kWaveRoundLimit) {
+      stuck = true;
+      break;
+    }
     const unsigned long long act = __ballot(my != kIdle);
     if (act != ~0ull) {
       if (pnext >= pend) {
@@ -310,21 +157,22 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
     if (__ballot(my != kIdle) == 0ull) break;
     if (my != kIdle) {
       const DevScene& S = B.jobs[uniform(v.ch[pci].job)].S;
-      if (w.step(S, st, c)) {
+      if (w.round(S, st, c)) {
         overflow |= st.overflow;
+        stuck |= w.stuck;
         finish(my, w);
         ++rays;
         my = kIdle;
       }
     }
   }
+  if (__ballot(stuck) != 0ull && lane_id() == 0) atomicAdd(&B.counters[kWatchdog], 1u);
 }
 
 // ---------------------------------------------------------------------------
 // closest: persistent closest-hit traversal of queue qi.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int cursor, int lds_depth) {
-  extern __shared__ uint32_t lds[];
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
   load_batch(v, B, counts, false);
@@ -333,7 +181,7 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
     B.counters[kShCnt + threadIdx.x] = 0u;
   }
   const PathQueue q = B.q[qi];
-  TravStack st = make_stack(lds, B.spill, lds_depth);
+  TravStack st = make_stack(B.spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
@@ -359,6 +207,7 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
 // (cpu_tiled.cpp:30-216) for every entry of queue qi at depth `depth`.
 // Survivors and shadow rays are appended to the same chunk's segments.
 // ---------------------------------------------------------------------------
+template <bool kGuided>
 __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth) {
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
@@ -381,6 +230,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
     float npdf = 0.f;
     uint32_t ndelta = 0u;
     uint32_t slot = 0u;
+    uint32_t nrec = kNoRecord;
     Rng rng{0};
     if (off < v.cnt[ci]) {
       const float4 r0 = q.ray0[i], r1 = q.ray1[i], th = q.thr[i];
@@ -391,6 +241,12 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
       f3 T{th.x, th.y, th.z};
       rng.s = static_cast<uint64_t>(ps.x) | (static_cast<uint64_t>(ps.y) << 32);
       slot = ps.z;
+      if (kGuided && ps.w < B.rec_cap) {
+        // the previous vertex recorded a training sample: note the path's
+        // radiance so far (its NEE included) so that what follows is known
+        const float4 a = B.l_nee[2 * slot], b = B.l_nee[2 * slot + 1];
+        B.rec[3 * B.rec_cap + ps.w] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, __uint_as_float(slot));
+      }
       const float4 hv = B.hit[i];
       HitRef h{hv.x, hv.y, hv.z, B.hit_slot[i], __float_as_int(hv.w)};
       if (h.inst < 0) {
@@ -424,6 +280,10 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
         } else {
           const DMat m = load_mat(S.mats, sf.mat);
           const bool delta = m.delta();
+          // guided vertex: one-sample MIS of the stored guide mixture and
+          // the BSDF sampler, with probabilities alpha and 1 - alpha
+          const float alpha = B.guide_alpha;
+          const bool guided = kGuided && lambert_exact(m);
           // area-light NEE (cpu_tiled.cpp:88-139)
           if (S.num_lights > 0 && !delta) {
             const int nl = S.num_lights;
@@ -451,7 +311,8 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               const f3 wi_l = fr.local(wi);
               if (cl > 1e-6f && wi_l.z > 1e-6f) {
                 const f3 f = nee_f(wo_l, wi_l, m);
-                const float bp = nee_pdf(wo_l, wi_l, m);
+                float bp = nee_pdf(wo_l, wi_l, m);
+                if (guided) bp = alpha * stored_pdf(B.mix, B.capacity, i, wi) + (1.f - alpha) * bp;
                 const float lpdf = ((1.0f / l0.w) * pick) * (d2 / cl);
                 const float w = mis_power2(lpdf, bp);
                 const float cs = wi_l.z;
@@ -480,7 +341,8 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               const f3 wi_l = fr.local(sdir);
               if (wi_l.z > 1e-6f) {
                 const f3 f = nee_f(wo_l, wi_l, m);
-                const float bp = nee_pdf(wo_l, wi_l, m);
+                float bp = nee_pdf(wo_l, wi_l, m);
+                if (guided) bp = alpha * stored_pdf(B.mix, B.capacity, i, sdir) + (1.f - alpha) * bp;
                 const float cs = wi_l.z;
                 const float lpdf = 1e8f * spick;
                 const float w = mis_power2(lpdf, bp);
@@ -500,7 +362,52 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               if (rng.f() > pm) alive = false;
               else T = T * (1.f / pm);
             }
-            if (alive) {
+            if (alive && guided) {
+              const Frame fs(sf.n);
+              const f3 wo_l = fs.local(-d);
+              f3 wi_l{0.f, 0.f, 0.f}, wiw{0.f, 0.f, 0.f};
+              if (rng.f() < alpha) {
+                const float u0 = rng.f();
+                const float u1 = rng.f();
+                const float u2 = rng.f();
+                wiw = stored_sample(B.mix, B.capacity, i, u0, u1, u2);
+                wi_l = fs.local(wiw);
+              } else {
+                BSample bs;
+                if (sample_bsdf(wo_l, m, rng, bs)) {
+                  wi_l = bs.wi;
+                  wiw = fs.world(bs.wi);
+                }
+              }
+              // exact Lambertian: f = base (1 - metallic) / pi on wo's side
+              // of the shading normal (the side the BSDF sampler draws on)
+              const bool same = (wo_l.z < 0.f) ? (wi_l.z < 0.f) : (wi_l.z > 0.f);
+              const float ci2 = fabsf(wi_l.z);
+              const float pc = alpha * stored_pdf(B.mix, B.capacity, i, wiw) + (1.f - alpha) * (same ? ci2 * kInvPiD : 0.f);
+              if (!same || ci2 < 1e-8f || !(pc >= 1e-8f)) {
+                alive = false;
+              } else {
+                if (B.rec_every > 0u && record_path(slot, v.batch, B.rec_every)) {
+                  const uint32_t r = atomicAdd(&B.counters[kRecCnt], 1u);
+                  if (r < B.rec_cap) {
+                    const uint32_t R = B.rec_cap;
+                    const float nan = __builtin_nanf("");
+                    B.rec[r] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, T.x);
+                    B.rec[R + r] = make_float4(sf.n.x, sf.n.y, sf.n.z, T.y);
+                    B.rec[2 * R + r] = make_float4(wiw.x, wiw.y, wiw.z, T.z);
+                    B.rec[3 * R + r] = make_float4(nan, nan, nan, __uint_as_float(slot));
+                    nrec = r;
+                  }
+                }
+                T = T * (m.base * ((1.f - m.metallic) * kInvPiD * ci2 / pc));
+                want_n = true;
+                no = offset_origin(sf.pos, sf.gn, wiw, kBounceOffset);
+                nd = wiw;
+                nT = T;
+                npdf = pc;
+                ndelta = 0u;
+              }
+            } else if (alive) {
               const Frame fs(sf.n);
               const f3 wo_l = fs.local(-d);
               BSample bs;
@@ -547,7 +454,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
       qn.ray0[in] = make_float4(no.x, no.y, no.z, nd.x);
       qn.ray1[in] = make_float4(nd.y, nd.z, npdf, __uint_as_float(ndelta));
       qn.thr[in] = make_float4(nT.x, nT.y, nT.z, 0.f);
-      qn.st[in] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), slot, 0u);
+      qn.st[in] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), slot, nrec);
     }
   }
 }
@@ -557,11 +464,10 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
 // to their path's per-slot NEE sum (one writer per (slot, light kind)).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int lds_depth) {
-  extern __shared__ uint32_t lds[];
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + kShCnt;
   load_batch(v, B, counts, false);
-  TravStack st = make_stack(lds, B.spill, lds_depth);
+  TravStack st = make_stack(B.spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
@@ -646,16 +552,19 @@ __global__ void __launch_bounds__(256) k_average(const float4* accum, float* out
 __global__ void __launch_bounds__(256) k_api_closest(DevScene S, const float4* rays, float* hits, int n,
                                                      uint32_t* spill, int lds_depth, unsigned long long* stats,
                                                      uint32_t* overflow_ctr) {
-  extern __shared__ uint32_t lds[];
-  TravStack st = make_stack(lds, spill, lds_depth);
+  TravStack st = make_stack(spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  uint32_t rays_done = 0;
-  if (i < n) {
-    const float4 a = rays[2 * i], b = rays[2 * i + 1];
-    const f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
-    HitRef h{0.f, 0.f, 0.f, 0u, -1};
-    const bool ok = walk_scene<false>(S, o, d, a.w, b.w, h, st, c);
+  const bool valid = i < n;
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 1.f, 0.f);
+  if (valid) {
+    a = rays[2 * i];
+    b = rays[2 * i + 1];
+  }
+  const f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+  HitRef h{0.f, 0.f, 0.f, 0u, -1};
+  const bool ok = walk_scene<false>(S, o, d, a.w, b.w, valid, h, st, c);
+  if (valid) {
     float* out = hits + 16 * static_cast<size_t>(i);
     if (!ok) {
       out[0] = __builtin_huge_valf();
@@ -679,27 +588,29 @@ __global__ void __launch_bounds__(256) k_api_closest(DevScene S, const float4* r
       out[12] = sf.gn.x; out[13] = sf.gn.y; out[14] = sf.gn.z;
       reinterpret_cast<int*>(out)[15] = sf.front ? 1 : 0;
     }
-    rays_done = 1;
     if (st.overflow) atomicAdd(overflow_ctr, 1u);
   }
-  flush_counts(stats, c, rays_done, false);
+  flush_counts(stats, c, valid ? 1u : 0u, false);
 }
 
 __global__ void __launch_bounds__(256) k_api_any(DevScene S, const float4* rays, uint8_t* occ, int n, uint32_t* spill,
                                                  int lds_depth, unsigned long long* stats, uint32_t* overflow_ctr) {
-  extern __shared__ uint32_t lds[];
-  TravStack st = make_stack(lds, spill, lds_depth);
+  TravStack st = make_stack(spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  uint32_t rays_done = 0;
-  if (i < n) {
-    const float4 a = rays[2 * i], b = rays[2 * i + 1];
-    HitRef h{0.f, 0.f, 0.f, 0u, -1};
-    occ[i] = walk_scene<true>(S, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, a.w, b.w, h, st, c) ? 1 : 0;
-    rays_done = 1;
+  const bool valid = i < n;
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 1.f, 0.f);
+  if (valid) {
+    a = rays[2 * i];
+    b = rays[2 * i + 1];
+  }
+  HitRef h{0.f, 0.f, 0.f, 0u, -1};
+  const bool hit = walk_scene<true>(S, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, a.w, b.w, valid, h, st, c);
+  if (valid) {
+    occ[i] = hit ? 1 : 0;
     if (st.overflow) atomicAdd(overflow_ctr, 1u);
   }
-  flush_counts(stats, c, rays_done, true);
+  flush_counts(stats, c, valid ? 1u : 0u, true);
 }
 
 // ---------------------------------------------------------------------------
@@ -741,7 +652,8 @@ hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int curso
 }
 
 hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, int grid) {
-  hipLaunchKernelGGL(k_shade, dim3(grid), dim3(256), 0, s, B, qi, depth);
+  if (B.guide_alpha > 0.f) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, B, qi, depth);
+  else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, s, B, qi, depth);
   return hipGetLastError();
 }
 

@@ -29,11 +29,15 @@ hipError_t launch_guide_eval(hipStream_t s, const GuideNet& net, int n, const fl
 hipError_t launch_guide_query(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
                               const float* u, float* dir, float* pdf);
 hipError_t launch_guide_train(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
-                              const float* dir, const float* lam, float lam_scale, float* grid_g, float* w_g,
-                              float* loss_sum);
+                              const float* dir, const float* lam, float lam_scale, const uint32_t* n_dev,
+                              const float* lam_sum_dev, float* grid_g, float* w_g, float* loss_sum);
 hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, float* grad, float* m, float* v,
                              size_t count, float lr, float b1, float b2, float eps, float bc1, float bc2,
-                             float grad_scale);
+                             float grad_scale, const uint32_t* n_dev);
+hipError_t launch_tile_weights(hipStream_t s, const uint16_t* w, uint16_t* wt);
+hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, const GuideNet& net, int grid);
+hipError_t launch_guide_collect(hipStream_t s, const WaveBuffers& B, float* pos, float* nrm, float* dir, float* lam,
+                                float* lam_sum, uint32_t* n_out);
 hipError_t launch_to_half(hipStream_t s, const float* src, uint16_t* dst, size_t count);
 
 }  // namespace xnh

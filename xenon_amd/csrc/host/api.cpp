@@ -52,9 +52,17 @@ struct GuideState {
   uint16_t* grid_h = nullptr;
   float *w_m = nullptr, *w_g = nullptr, *w_am = nullptr, *w_av = nullptr;
   uint16_t* w_h = nullptr;
+  uint16_t* w_t = nullptr;        // FP16 weights as MFMA operand tiles
   float* loss = nullptr;
   size_t ngrid = 0;
   int step = 0;
+  // guided rendering: per-entry mixtures, training records and the device
+  // training batch built from them
+  float4* mix = nullptr;
+  float4* rec = nullptr;
+  float *t_pos = nullptr, *t_nrm = nullptr, *t_dir = nullptr, *t_lam = nullptr, *t_lam_sum = nullptr;
+  uint32_t* t_n = nullptr;
+  bool train = true;
 };
 
 struct xnh_ctx {
@@ -453,6 +461,7 @@ int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
+    if (c->guide.on) HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, c->guide.net, c->grid_shade * 2));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     HIP_OR_FAIL(c, launch_shade(s, c->B, qi, d, c->grid_shade));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
@@ -488,6 +497,30 @@ int ensure_graph(xnh_ctx* c, int depth) {
   }
   c->graph_depth = depth;
   c->graph_gen = c->table_gen;
+  return XNH_OK;
+}
+
+// One guiding step on the batch just rendered: records -> training batch
+// (all counts stay on the device) -> fused forward/backward -> Adam.
+int guide_train_batch(xnh_ctx* c) {
+  GuideState& g = c->guide;
+  hipStream_t s = c->stream;
+  const int R = static_cast<int>(c->B.rec_cap);
+  HIP_OR_FAIL(c, hipMemsetAsync(g.t_lam_sum, 0, sizeof(float), s));
+  HIP_OR_FAIL(c, launch_guide_collect(s, c->B, g.t_pos, g.t_nrm, g.t_dir, g.t_lam, g.t_lam_sum, g.t_n));
+  HIP_OR_FAIL(c, hipMemsetAsync(g.loss, 0, sizeof(float), s));
+  HIP_OR_FAIL(c, launch_guide_train(s, g.net, R, g.t_pos, g.t_nrm, g.t_dir, g.t_lam, 1.f, g.t_n, g.t_lam_sum, g.grid_g,
+                                    g.w_g, g.loss));
+  g.step++;
+  const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
+  const float bc1 = 1.f - static_cast<float>(std::pow(double(b1), g.step));
+  const float bc2 = 1.f - static_cast<float>(std::pow(double(b2), g.step));
+  const float lr = g.cfg.learning_rate;
+  HIP_OR_FAIL(c, launch_guide_adam(s, g.grid_m, g.grid_h, g.grid_g, g.grid_am, g.grid_av, g.ngrid, lr, b1, b2, eps, bc1,
+                                   bc2, 1.f, g.t_n));
+  HIP_OR_FAIL(c, launch_guide_adam(s, g.w_m, g.w_h, g.w_g, g.w_am, g.w_av, size_t(kWTotal), lr, b1, b2, eps, bc1, bc2,
+                                   1.f, g.t_n));
+  HIP_OR_FAIL(c, launch_tile_weights(s, g.w_h, g.w_t));
   return XNH_OK;
 }
 
@@ -585,12 +618,18 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
       rc = enqueue_batch(c, depth, c->profiling ? &ev : nullptr);
       if (rc != XNH_OK) return rc;
     }
+    if (c->guide.on && c->guide.train) {
+      rc = guide_train_batch(c);
+      if (rc != XNH_OK) return rc;
+    }
   }
   HIP_OR_FAIL(c, hipEventRecord(c->ev_end, s));
   unsigned long long st[kNumStat];
   uint32_t ovf = 0;
   HIP_OR_FAIL(c, hipMemcpyAsync(st, c->B.stats, sizeof st, hipMemcpyDeviceToHost, s));
   HIP_OR_FAIL(c, hipMemcpyAsync(&ovf, c->B.counters + kOverflow, sizeof ovf, hipMemcpyDeviceToHost, s));
+  uint32_t dog = 0;
+  HIP_OR_FAIL(c, hipMemcpyAsync(&dog, c->B.counters + kWatchdog, sizeof dog, hipMemcpyDeviceToHost, s));
   HIP_OR_FAIL(c, hipStreamSynchronize(s));
   xnh_stats& o = c->stats;
   o = xnh_stats{};
@@ -622,6 +661,7 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
       o.shadow_ms += d;
     }
   }
+  if (dog) return fail(c, XNH_E_HIP, "traversal watchdog: " + std::to_string(dog) + " wave(s) reached the iteration bound");
   if (ovf) return fail(c, XNH_E_OVERFLOW, "traversal stack overflow");
   return XNH_OK;
 }
@@ -837,6 +877,95 @@ int xnh_reset(xnh_ctx* c) {
   return XNH_OK;
 }
 
+// ---- tuning -------------------------------------------------------------------
+
+int xnh_set_tuning(xnh_ctx* c, int32_t block_size, int32_t lds_stack, int32_t spp_per_launch) {
+  if (!c) return XNH_E_INVALID;
+  const int block = block_size > 0 ? block_size : 256;
+  const int lds = lds_stack > 0 ? lds_stack : 16;
+  if (block % 64 != 0 || block > 256 || lds > kMaxStack || spp_per_launch < 0)
+    return fail(c, XNH_E_INVALID, "tuning out of range (block 64..256 in steps of 64, lds_stack 1..128)");
+  const long long npix = static_cast<long long>(c->cfg.width) * c->cfg.height;
+  int ns = spp_per_launch;
+  if (ns == 0) ns = static_cast<int>(std::max(1LL, std::min(64LL, (8LL << 20) / npix)));
+  if (npix * ns > (1LL << 30)) return fail(c, XNH_E_INVALID, "queue capacity above 2^30 paths");
+  if (c->guide.on && ns != c->nsamples) return fail(c, XNH_E_INVALID, "queue capacity is fixed while guiding is on");
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  const int grid = trace_grid(block, lds);
+  if (grid <= 0) return fail(c, XNH_E_HIP, "occupancy query failed");
+  WaveBuffers keep = c->B;
+  c->block = block;
+  c->lds_depth = lds;
+  c->nsamples = ns;
+  c->grid_trace = grid;
+  const int rc = alloc_waves(c);
+  if (rc != XNH_OK) return rc;
+  c->B.mix = keep.mix;
+  c->B.rec = keep.rec;
+  c->B.rec_cap = keep.rec_cap;
+  c->B.rec_every = keep.rec_every;
+  c->B.guide_alpha = keep.guide_alpha;
+  c->spp_total = 0;
+  c->table_gen++;   // the captured graph holds the old buffers
+  return XNH_OK;
+}
+
+int xnh_get_tuning(const xnh_ctx* c, int32_t* block_size, int32_t* lds_stack, int32_t* spp_per_launch) {
+  if (!c) return XNH_E_INVALID;
+  if (block_size) *block_size = c->block;
+  if (lds_stack) *lds_stack = c->lds_depth;
+  if (spp_per_launch) *spp_per_launch = c->nsamples;
+  return XNH_OK;
+}
+
+int xnh_autotune(xnh_ctx* c, const xnh_camera* cam, int32_t probe_spp, xnh_tune_result* out) {
+  if (!c || !cam || probe_spp <= 0) return XNH_E_INVALID;
+  if (!c->has_scene) return fail(c, XNH_E_NO_SCENE, "xnh_autotune before xnh_upload_scene");
+  const long long npix = static_cast<long long>(c->cfg.width) * c->cfg.height;
+  const int ns0 = static_cast<int>(std::max(1LL, std::min(64LL, (8LL << 20) / npix)));
+  std::vector<int> caps = {ns0};
+  if (!c->guide.on) {
+    if (ns0 / 2 >= 1) caps.push_back(ns0 / 2);
+    if (npix * ns0 * 2 <= (1LL << 30) && ns0 * 2 <= 128) caps.push_back(ns0 * 2);
+  }
+  const int blocks[] = {64, 128, 256};
+  const int ldss[] = {8, 12, 16, 24, 32};
+  float best = 3.0e38f;
+  int bb = c->block, bl = c->lds_depth, bn = c->nsamples, tried = 0;
+  const bool prof = c->profiling;
+  c->profiling = false;
+  for (int ns : caps)
+    for (int b : blocks)
+      for (int l : ldss) {
+        int rc = xnh_set_tuning(c, b, l, ns);
+        if (rc != XNH_OK) continue;
+        // one untimed batch, then the probe
+        rc = xnh_render(c, cam, 1, std::min(ns, probe_spp), nullptr);
+        if (rc == XNH_OK) rc = xnh_render(c, cam, 1, probe_spp, nullptr);
+        if (rc != XNH_OK) continue;
+        ++tried;
+        const float ms = static_cast<float>(c->stats.render_ms) / static_cast<float>(probe_spp);
+        if (ms < best) {
+          best = ms;
+          bb = b;
+          bl = l;
+          bn = ns;
+        }
+      }
+  c->profiling = prof;
+  const int rc = xnh_set_tuning(c, bb, bl, bn);
+  if (rc != XNH_OK) return rc;
+  if (tried == 0) return fail(c, XNH_E_HIP, "no tuning candidate rendered");
+  if (out) {
+    out->block_size = bb;
+    out->lds_stack = bl;
+    out->spp_per_launch = bn;
+    out->ms_per_spp = best;
+    out->candidates = tried;
+  }
+  return XNH_OK;
+}
+
 int xnh_spp(const xnh_ctx* c) { return c ? c->spp_total : XNH_E_INVALID; }
 
 int xnh_get_stats(const xnh_ctx* c, xnh_stats* out) {
@@ -859,6 +988,18 @@ int xnh_guide_disable(xnh_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   free_all(c->guide.allocs);
   c->guide = GuideState{};
+  c->B.mix = nullptr;
+  c->B.rec = nullptr;
+  c->B.rec_cap = 0;
+  c->B.rec_every = 0;
+  c->B.guide_alpha = 0.f;
+  c->table_gen++;
+  return XNH_OK;
+}
+
+int xnh_guide_set_training(xnh_ctx* c, int32_t on) {
+  if (!c || !c->guide.on) return XNH_E_INVALID;
+  c->guide.train = on != 0;
   return XNH_OK;
 }
 
@@ -886,13 +1027,31 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_am, size_t(kWTotal));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_av, size_t(kWTotal));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_h, size_t(kWTotal));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_t, size_t(kWTotal));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.loss, size_t(1));
+  const size_t cap = static_cast<size_t>(c->B.capacity);
+  const size_t R = static_cast<size_t>(cfg.train_batch);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.mix, size_t(kMixVecs) * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.rec, 4 * R);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_pos, 3 * R);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_nrm, 3 * R);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_dir, 3 * R);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_lam, R);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_lam_sum, size_t(1));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_n, size_t(1));
   if (rc != XNH_OK) {
     xnh_guide_disable(c);
     return rc;
   }
   g.net.grid = g.grid_h;
   g.net.w = g.w_h;
+  g.net.wt = g.w_t;
+  c->B.mix = g.mix;
+  c->B.rec = g.rec;
+  c->B.rec_cap = static_cast<uint32_t>(R);
+  c->B.rec_every = static_cast<uint32_t>(std::max<size_t>(1, cap / static_cast<size_t>(cfg.record_paths)));
+  c->B.guide_alpha = cfg.guide_fraction;
+  c->table_gen++;   // the captured batch graph gains the guide kernels
   g.net.t_log2 = cfg.t_log2;
   const double b = std::exp((std::log(2048.0) - std::log(16.0)) / (kGLevels - 1));
   for (int l = 0; l < kGLevels; ++l) g.net.res[l] = static_cast<int32_t>(std::floor(16.0 * std::pow(b, l)));
@@ -924,6 +1083,7 @@ int xnh_guide_set_params(xnh_ctx* c, const float* grid, const float* w) {
   HIP_OR_FAIL(c, hipMemcpyAsync(g.w_m, w, size_t(kWTotal) * 4, hipMemcpyHostToDevice, s));
   HIP_OR_FAIL(c, launch_to_half(s, g.grid_m, g.grid_h, g.ngrid));
   HIP_OR_FAIL(c, launch_to_half(s, g.w_m, g.w_h, size_t(kWTotal)));
+  HIP_OR_FAIL(c, launch_tile_weights(s, g.w_h, g.w_t));
   for (float* p : {g.grid_g, g.grid_am, g.grid_av}) HIP_OR_FAIL(c, hipMemsetAsync(p, 0, g.ngrid * 4, s));
   for (float* p : {g.w_g, g.w_am, g.w_av}) HIP_OR_FAIL(c, hipMemsetAsync(p, 0, size_t(kWTotal) * 4, s));
   HIP_OR_FAIL(c, hipStreamSynchronize(s));
@@ -997,7 +1157,7 @@ static int guide_step(xnh_ctx* c, int n, const float* dp, const float* dn, const
   GuideState& g = c->guide;
   hipStream_t s = c->stream;
   HIP_OR_FAIL(c, hipMemsetAsync(g.loss, 0, 4, s));
-  HIP_OR_FAIL(c, launch_guide_train(s, g.net, n, dp, dn, dd, dl, lam_scale, g.grid_g, g.w_g, g.loss));
+  HIP_OR_FAIL(c, launch_guide_train(s, g.net, n, dp, dn, dd, dl, lam_scale, nullptr, nullptr, g.grid_g, g.w_g, g.loss));
   if (apply) {
     g.step++;
     const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
@@ -1005,9 +1165,10 @@ static int guide_step(xnh_ctx* c, int n, const float* dp, const float* dn, const
     const float bc2 = 1.f - static_cast<float>(std::pow(double(b2), g.step));
     const float lr = g.cfg.learning_rate, gs = 1.f / static_cast<float>(n);
     HIP_OR_FAIL(c, launch_guide_adam(s, g.grid_m, g.grid_h, g.grid_g, g.grid_am, g.grid_av, g.ngrid, lr, b1, b2, eps,
-                                     bc1, bc2, gs));
+                                     bc1, bc2, gs, nullptr));
     HIP_OR_FAIL(c, launch_guide_adam(s, g.w_m, g.w_h, g.w_g, g.w_am, g.w_av, size_t(kWTotal), lr, b1, b2, eps, bc1,
-                                     bc2, gs));
+                                     bc2, gs, nullptr));
+    HIP_OR_FAIL(c, launch_tile_weights(s, g.w_h, g.w_t));
   }
   return XNH_OK;
 }

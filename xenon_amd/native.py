@@ -117,6 +117,11 @@ class GuideConfig(C.Structure):
                 ("train_batch", C.c_int32), ("record_paths", C.c_int32)]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("block_size", C.c_int32), ("lds_stack", C.c_int32), ("spp_per_launch", C.c_int32),
+                ("ms_per_spp", C.c_float), ("candidates", C.c_int32)]
+
+
 _vp = C.c_void_p
 SIGNATURES = {
     "xnh_create": (C.c_int, [C.POINTER(Config), C.POINTER(_vp)]),
@@ -145,8 +150,12 @@ SIGNATURES = {
     "xnh_scene_build": (C.c_int, [_vp, C.c_int32]),
     "xnh_scene_flat": (C.c_int, [_vp, C.POINTER(FlatScene)]),
     "xnh_camera_look_at": (C.c_int, [_fp, _fp, _fp, C.c_float, C.c_float, C.POINTER(Camera)]),
+    "xnh_set_tuning": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
+    "xnh_get_tuning": (C.c_int, [_vp, _ip, _ip, _ip]),
+    "xnh_autotune": (C.c_int, [_vp, C.POINTER(Camera), C.c_int32, C.POINTER(TuneResult)]),
     "xnh_guide_enable": (C.c_int, [_vp, C.POINTER(GuideConfig)]),
     "xnh_guide_disable": (C.c_int, [_vp]),
+    "xnh_guide_set_training": (C.c_int, [_vp, C.c_int32]),
     "xnh_guide_set_params": (C.c_int, [_vp, _fp, _fp]),
     "xnh_guide_get_params": (C.c_int, [_vp, _fp, _fp]),
     "xnh_guide_eval": (C.c_int, [_vp, C.c_int32, _fp, _fp, _fp]),

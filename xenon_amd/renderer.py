@@ -121,6 +121,15 @@ class HipScene:
         return nodes, order, xf, ids, meshes
 
 
+    def bounds(self):
+        """World bounding box of the scene (the TLAS root box): (lo, hi)."""
+        f = self._flat
+        if f.num_tlas_nodes == 0:
+            return np.zeros(3, np.float32), np.ones(3, np.float32)
+        root = np.ctypeslib.as_array(C.cast(f.tlas_nodes, C.POINTER(C.c_uint8)), shape=(32,)).view(NODE_DTYPE)[0]
+        return np.array(root["lo"], np.float32), np.array(root["hi"], np.float32)
+
+
 class GpuScene:
     """A scene kept resident on the device for batched jobs (xnh_scene_upload)."""
 
@@ -229,13 +238,72 @@ class HipRenderer:
         self._chk(N.lib().xnh_render_jobs(self.h, arr, len(jobs)))
         return (outs, sums) if want_sums else outs
 
+    # -- tuning -----------------------------------------------------------------
+    def set_tuning(self, block_size: int = 0, lds_stack: int = 0, spp_per_launch: int = 0):
+        """Traversal workgroup size, LDS stack entries per lane and samples per
+        batch (queue capacity); 0 = default. Clears the accumulation."""
+        self._chk(N.lib().xnh_set_tuning(self.h, int(block_size), int(lds_stack), int(spp_per_launch)))
+
+    def tuning(self) -> dict:
+        b, l, n = C.c_int32(), C.c_int32(), C.c_int32()
+        self._chk(N.lib().xnh_get_tuning(self.h, C.byref(b), C.byref(l), C.byref(n)))
+        return {"block_size": b.value, "lds_stack": l.value, "spp_per_launch": n.value}
+
+    def autotune(self, camera: N.Camera, probe_spp: int = 8, cache: str | None = "default") -> dict:
+        """Picks the fastest (workgroup size, LDS stack depth, queue capacity)
+        for the uploaded scene by timing every candidate on `probe_spp`
+        samples, or reuses the choice cached for the same scene shape,
+        resolution and depth (cache=None: always measure). Clears the
+        accumulation."""
+        import json
+        import os
+        key = None
+        path = None
+        if cache is not None and self.scene is not None:
+            path = (os.path.join(os.path.expanduser("~"), ".cache", "xenon_amd", "autotune.json")
+                    if cache == "default" else cache)
+            f = self.scene._flat
+            key = (f"{self.width}x{self.height}:tris{sum(f.meshes[m].num_triangles for m in range(f.num_meshes))}"
+                   f":inst{f.num_instances}:probe{probe_spp}")
+            try:
+                with open(path) as fh:
+                    hit = json.load(fh).get(key)
+            except (OSError, ValueError):
+                hit = None
+            if hit:
+                self.set_tuning(hit["block_size"], hit["lds_stack"], hit["spp_per_launch"])
+                return dict(hit, cached=True)
+        res = N.TuneResult()
+        self._chk(N.lib().xnh_autotune(self.h, C.byref(camera), int(probe_spp), C.byref(res)))
+        out = {"block_size": res.block_size, "lds_stack": res.lds_stack, "spp_per_launch": res.spp_per_launch,
+               "ms_per_spp": res.ms_per_spp, "candidates": res.candidates}
+        if key is not None:
+            try:
+                os.makedirs(os.path.dirname(path), exist_ok=True)
+                try:
+                    with open(path) as fh:
+                        db = json.load(fh)
+                except (OSError, ValueError):
+                    db = {}
+                db[key] = out
+                with open(path, "w") as fh:
+                    json.dump(db, fh, indent=1)
+            except OSError:
+                pass
+        return dict(out, cached=False)
+
     # -- path guiding -----------------------------------------------------------
     GUIDE_LEVELS, GUIDE_FEATURES, GUIDE_WEIGHTS = 16, 2, 48 * 64 + 64 * 64 + 64 * 64 + 64 * 48
 
-    def guide_enable(self, bounds_lo, bounds_hi, t_log2: int = 19, learning_rate: float = 5e-3,
+    def guide_enable(self, bounds_lo=None, bounds_hi=None, t_log2: int = 19, learning_rate: float = 5e-3,
                      guide_fraction: float = 0.5, seed: int = 0, train_batch: int = 0, record_paths: int = 0):
         """Turns learned path guiding on with fresh parameters; positions
-        inside [bounds_lo, bounds_hi] are mapped to the hash grid."""
+        inside [bounds_lo, bounds_hi] (default: the uploaded scene's box)
+        are mapped to the hash grid."""
+        if bounds_lo is None or bounds_hi is None:
+            if self.scene is None:
+                raise ValueError("guide_enable needs bounds or an uploaded scene")
+            bounds_lo, bounds_hi = self.scene.bounds()
         cfg = N.GuideConfig(int(t_log2), float(learning_rate), float(guide_fraction), int(seed) & 0xFFFFFFFF,
                             (C.c_float * 3)(*bounds_lo), (C.c_float * 3)(*bounds_hi), int(train_batch),
                             int(record_paths))
@@ -244,6 +312,10 @@ class HipRenderer:
 
     def guide_disable(self):
         self._chk(N.lib().xnh_guide_disable(self.h))
+
+    def guide_set_training(self, on: bool):
+        """Train after every rendered batch (default) or keep the network frozen."""
+        self._chk(N.lib().xnh_guide_set_training(self.h, 1 if on else 0))
 
     def guide_params(self):
         """FP32 master parameters: (grid (16, T, 2), weights (12288,))."""
