@@ -13,8 +13,6 @@ def _relmse(x, r):
     return float(np.mean((x[ok] - r[ok]) ** 2 / (r[ok] ** 2 + 1e-2)))
 
 
-@pytest.mark.skip(reason="held off the GPU: a full-suite run was stopped for silence while this test ran "
-                         "(after the guided-rendering tests in the same process) and no cause has been shown yet")
 def test_jobs_equal_single_renders(oracle):
     from xenon_amd import GpuScene, HipRenderer, HipScene, scenes
     specs = [scenes.Job(0, 48, 32, 12, 6, 1), scenes.Job(1, 40, 24, 5, 8, 2), scenes.Job(2, 32, 32, 7, 10, 3),

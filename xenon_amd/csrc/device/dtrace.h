@@ -132,25 +132,23 @@ __device__ __forceinline__ InstRec load_inst(const float4* inst, int i) {
 // same lane while the rest of the wave keeps walking.
 //
 // A round runs the lane's TLAS steps until it enters a BLAS walk, then two
-// wave-wide phases over the lanes inside BLAS walks (Aila and Laine's
-// while-while with speculative traversal): an interior phase that walks
-// node pairs, continuing with the nearer hit child in registers and
-// pushing only the farther, and a leaf phase that tests triangles. A lane
-// that reaches a leaf postpones it (one slot) and keeps walking interior
-// nodes until every lane holds a leaf, so both phases run with most lanes
-// busy.
+// wave-wide phases over the lanes inside BLAS walks (the while-while loop of
+// Aila and Laine): an interior phase that walks node pairs, continuing with
+// the nearer hit child in registers and pushing only the farther, until the
+// lane reaches a leaf (held in a slot), and a leaf phase that tests the held
+// leaves, continuing through consecutive leaves. Each phase runs one kind
+// of step for all its lanes instead of both kinds every iteration.
 //
-// Results equal the reference's recursive walk, ties included: triangles
-// are still tested in depth-first order (a postponed leaf is tested before
-// any later leaf), and interior nodes tested before a postponed leaf see a
-// t_best that is too large only by that leaf's hits, which can add visits
-// (counted) but never an accepted triangle (a subtree the reference culls
-// has t > its t_best at that point, which is >= ours when the subtree's
-// triangles are tested). The visiting order otherwise follows the
-// reference: BLAS children nearer entry first, ties right first (closest) /
-// left first (any); TLAS test-on-pop with centre-ordered children
-// (tlas.cpp:231-241); a TLAS leaf's instances (pushed with kInstTag, in
-// reverse) walked first to last. Closest hit starts from t_best = +inf and
+// No step is taken early: a lane holding a leaf walks no further interior
+// nodes until the leaf is tested, since a node box tested against a t_best
+// not yet lowered by the leaf's hits may let through a subtree the reference
+// culls, and box and triangle rounding differ enough for such a subtree to
+// hold a hit. Every box and triangle test therefore sees the reference's
+// t_best, and results and work counts equal its recursive walk, ties
+// included. The visiting order follows the reference: BLAS children nearer
+// entry first, ties right first (closest) / left first (any); TLAS
+// test-on-pop with centre-ordered children (tlas.cpp:231-241); a TLAS leaf's
+// instances (pushed with kInstTag, in reverse) walked first to last. Closest hit starts from t_best = +inf and
 // ignores tmax, as Scene::intersect does with a default HitRecord; any hit
 // uses [tmin, tmax]. A stack overflow ends the ray with what it has found.
 constexpr uint32_t kInstTag = 0x80000000u;
@@ -166,8 +164,8 @@ struct Walker {
   float tmin, tbest;
   int blas_base;      // stack depth under the current BLAS walk; -1 in the TLAS
   int inst;           // instance of the BLAS walk
-  uint32_t e;         // next BLAS entry to process, or kEntEnd
-  uint32_t leaf;      // postponed leaf entry, 0 = none
+  uint32_t e;         // next BLAS entry to process, or kEntEnd; a leaf only while `leaf` is held
+  uint32_t leaf;      // held leaf entry (tested before e), 0 = none
   bool inst_found, found;
   bool stuck;         // an iteration bound was reached (the walk was abandoned)
   HitRef hit;
@@ -220,8 +218,14 @@ struct Walker {
         inst = ii;
         inst_found = false;
         blas_base = st.sp;
-        e = in.root;   // kNoRoot (an empty mesh) is kEntEnd
+        // e never holds a leaf unless one is already held: a root that is a
+        // leaf is held at once (kNoRoot, an empty mesh, is kEntEnd)
+        e = in.root;
         leaf = 0u;
+        if (ent_leaf(e)) {
+          leaf = e;
+          e = kEntEnd;
+        }
         return false;
       }
       const uint32_t ni = te;
@@ -264,9 +268,9 @@ struct Walker {
     return false;
   }
 
-  // One interior step on e (an interior entry): pair-test its children,
-  // continue with the nearer hit one, push the farther; a leaf reached
-  // while no leaf is postponed is postponed.
+  // One interior step on e (an interior entry, no leaf held): pair-test its
+  // children, continue with the nearer hit one, push the farther; a leaf
+  // reached is held and the next entry popped (walked after the leaf).
   __device__ __forceinline__ void interior_step(const DevScene& S, TravStack& st, TravCount& c) {
     const uint32_t child = e;
     const float4 la = S.bnodes[2 * child], lb = S.bnodes[2 * child + 1];
@@ -288,14 +292,14 @@ struct Walker {
     } else {
       next = pop_blas(st);
     }
-    if (leaf == 0u && ent_leaf(next)) {
+    if (ent_leaf(next)) {
       leaf = next;
       next = pop_blas(st);
     }
     e = next;
   }
 
-  // Tests the postponed leaf; true when an any-hit walk found an occluder.
+  // Tests the held leaf; true when an any-hit walk found an occluder.
   __device__ __forceinline__ bool leaf_step(const DevScene& S, TravStack& st, TravCount& c) {
     uint32_t count = leaf >> kEntShift, first = leaf & kEntIndexMask;
     if (count == kEntBigLeaf) {
@@ -333,9 +337,8 @@ struct Walker {
   __device__ __forceinline__ bool round(const DevScene& S, TravStack& st, TravCount& c) {
     if (tlas_steps(S, st, c)) return true;
     for (uint32_t guard = 0;; ++guard) {
-      const bool walk = ent_interior(e);
-      // stop when no lane can walk or every walking lane holds a leaf
-      if (__ballot(walk && leaf == 0u) == 0ull) break;
+      const bool walk = leaf == 0u && ent_interior(e);
+      if (__ballot(walk) == 0ull) break;
       if (guard >= kPhaseStepLimit) {
         stuck = true;
         return true;
