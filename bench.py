@@ -14,9 +14,12 @@ K timed steps bracketed by device synchronisation. `value` = pixels x samples
 
 roofline: the closest-hit traversal kernel. Its algorithmic bytes per ray are
 32 B ray in + 16 B hit out, plus 32 B per BVH node box tested, 36 B per
-triangle test and 96 B per instance visit (counted on the device for exactly
-the rays traced); achieved = those bytes / the kernel's summed launch time
-measured with HIP events on the library's stream during the timed steps.
+triangle test and 96 B per instance visit; achieved = those bytes / the
+kernel's summed launch time measured with HIP events on the library's stream
+during the timed steps. The timed steps run the production kernels, which
+keep no work counters; the node / triangle / instance counts come from
+rendering the same samples again (untimed) through the counting variants of
+the traversal kernels, which trace exactly the same rays.
 
 guided: the guided-vs-plain comparison at equal wall-clock time on the
 caustic room (a 499,280-triangle delta-glass sphere in a closed diffuse room
@@ -215,11 +218,13 @@ def main():
         r.render(cam, spp, sps, want_image=False)
         spp += sps
         log(f"warmup step {w + 1}/{args.warmup}: {r.stats()['render_ms']:.1f} ms")
-    r.set_profiling(True)
-    keys = ["closest_rays", "shadow_rays", "node_tests", "interior", "tri_tests", "inst_visits",
-            "shadow_node_tests", "shadow_interior", "shadow_tri_tests", "shadow_inst_visits", "nonfinite",
-            "closest_ms", "shadow_ms", "shade_ms", "closest_launches", "render_ms"]
-    tot = {k: 0 for k in keys}
+    r.set_profiling(timing=True)
+    keys = ["closest_rays", "shadow_rays", "nonfinite", "closest_ms", "shadow_ms", "shade_ms", "closest_launches",
+            "render_ms"]
+    work_keys = ["node_tests", "interior", "tri_tests", "inst_visits", "shadow_node_tests", "shadow_interior",
+                 "shadow_tri_tests", "shadow_inst_visits"]
+    tot = {k: 0 for k in keys + work_keys}
+    first_timed = spp
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
@@ -231,7 +236,17 @@ def main():
         log(f"step {k + 1}/{args.steps}: {st['render_ms']:.1f} ms device")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
-    r.set_profiling(False)
+    # Work counts of the timed steps: the same samples again through the
+    # counting kernel variants (every path is a function of its (pixel,
+    # sample) seed, so the rays, and the nodes and triangles they test, are
+    # identical); untimed.
+    r.set_profiling(work=True)
+    for k in range(args.steps):
+        r.render(cam, first_timed + k * sps, sps, want_image=False)
+        st = r.stats()
+        for key in work_keys:
+            tot[key] += st[key]
+    r.set_profiling()
     paths = desc.width * desc.height * sps * args.steps
     mpaths = paths / elapsed / 1e6
     mrays = (tot["closest_rays"] + tot["shadow_rays"]) / elapsed / 1e6

@@ -233,6 +233,8 @@ typedef struct {
   float bounds_lo[3], bounds_hi[3];   /* scene box mapped to the unit cube */
   int32_t train_batch;     /* training samples per step (0: 2^18) */
   int32_t record_paths;    /* paths per batch that record training samples (0: 2^16) */
+  int32_t train_steps;     /* optimisation steps per batch, each on an equal slice of its records (0: 1) */
+  int32_t warmup_steps;    /* the guide's MIS probability ramps from 0 to guide_fraction over this many steps */
 } xnh_guide_config;
 
 /* Turns guided rendering on for this context with fresh parameters. From
@@ -286,8 +288,16 @@ int xnh_read_accum(xnh_ctx* ctx, float* sum_out, int32_t* spp_out);
 int xnh_reset(xnh_ctx* ctx);
 int xnh_spp(const xnh_ctx* ctx);
 int xnh_get_stats(const xnh_ctx* ctx, xnh_stats* out);
-/* Turns per-kernel HIP-event timing on (1) or off (0, default). */
-int xnh_set_profiling(xnh_ctx* ctx, int32_t on);
+/* Profiling of renders, a mask of the flags below (0 = off, the default):
+ * XNH_PROFILE_TIMING records HIP events around every traversal and shading
+ * kernel (stats closest_ms / shade_ms / shadow_ms; the batch sequence is then
+ * launched directly instead of as a graph); XNH_PROFILE_WORK runs the
+ * counting variants of the traversal kernels (stats node_tests, interior,
+ * tri_tests, inst_visits and their shadow counterparts). Ray and path counts
+ * are always kept. */
+#define XNH_PROFILE_TIMING 1
+#define XNH_PROFILE_WORK 2
+int xnh_set_profiling(xnh_ctx* ctx, int32_t flags);
 
 /* ---- ray-batch parity API ----------------------------------------------- */
 /* Scene::intersect / Scene::intersects (scene/scene.h:126-131) for a batch

@@ -41,11 +41,12 @@ constexpr uint32_t kEntShift = 28;
 constexpr uint32_t kEntIndexMask = (1u << kEntShift) - 1u;
 constexpr uint32_t kEntBigLeaf = 15;
 constexpr uint32_t kNoRoot = 0xFFFFFFFFu;
-// Iteration bounds of the traversal loops: far above any legitimate walk
-// (a wave's whole share of a batch takes ~1e5 rounds), they guarantee every
-// wave exits; a wave that reaches one abandons its work and is reported.
+// Bound on the walk rounds of one wave in a launch: far above any
+// legitimate walk (a wave's share of a batch takes ~1e5 rounds), it makes
+// every wave exit even if a round stopped making progress; a wave that
+// reaches it abandons its work and is reported. (Within a round every loop
+// ends: the host rejects trees whose children do not follow their parent.)
 constexpr uint32_t kWaveRoundLimit = 1u << 26;
-constexpr uint32_t kPhaseStepLimit = 1u << 22;
 
 struct DevSky {
   float zenith[3], horizon[3], ground[3];
@@ -128,7 +129,8 @@ struct WaveBuffers {
   float4* rec;      // [4][rec_cap] training records, see guide.h
   uint32_t rec_cap;
   uint32_t rec_every;   // a path records its vertices when hash(slot, batch) % rec_every == 0
-  float guide_alpha;    // one-sample MIS probability of sampling the guide; 0 = off
+  float guide_alpha;    // largest one-sample MIS probability of the guide; 0 = guiding off
+  const float* alpha_now;   // the probability used by the current batch (device memory)
 };
 
 enum Ctr : int {

@@ -52,6 +52,9 @@ struct TravStack {
   }
 };
 
+// Traversal work counts. The rendering kernels have a counting variant
+// (the same walk with these increments) used when work counts are asked
+// for; the timed variant carries none of them.
 struct TravCount {
   uint32_t node_tests, interior, tri_tests, inst_visits;
 };
@@ -157,7 +160,7 @@ constexpr uint32_t kEntEnd = kNoRoot;   // no entry left in the BLAS walk
 __device__ __forceinline__ bool ent_interior(uint32_t e) { return e != kEntEnd && (e >> kEntShift) == 0u; }
 __device__ __forceinline__ bool ent_leaf(uint32_t e) { return e != kEntEnd && (e >> kEntShift) != 0u; }
 
-template <bool kAny>
+template <bool kAny, bool kCount = true>
 struct Walker {
   f3 o, d, inv;       // world ray
   f3 lo, ld, linv;    // instance-space ray of the BLAS walk in progress
@@ -167,8 +170,8 @@ struct Walker {
   uint32_t e;         // next BLAS entry to process, or kEntEnd; a leaf only while `leaf` is held
   uint32_t leaf;      // held leaf entry (tested before e), 0 = none
   bool inst_found, found;
-  bool stuck;         // an iteration bound was reached (the walk was abandoned)
-  HitRef hit;
+  bool stuck;         // the round bound was reached (the walk was abandoned)
+  HitRef hit;         // u, v, slot, inst of the closest hit; its t is tbest
 
   __device__ __forceinline__ void begin(const DevScene& S, f3 o_, f3 d_, float tmin_, float tmax_, TravStack& st) {
     o = o_;
@@ -201,17 +204,13 @@ struct Walker {
 
   // TLAS steps until this lane is inside a BLAS walk; true = ray finished.
   __device__ __forceinline__ bool tlas_steps(const DevScene& S, TravStack& st, TravCount& c) {
-    for (uint32_t guard = 0; blas_base < 0; ++guard) {
-      if (guard >= kPhaseStepLimit) {
-        stuck = true;
-        return true;
-      }
+    while (blas_base < 0) {
       if (st.sp == 0) return true;
       const uint32_t te = st.pop();
       if (te & kInstTag) {
         const int ii = static_cast<int>(te & ~kInstTag);
         const InstRec in = load_inst(S.inst, ii);
-        ++c.inst_visits;
+        if (kCount) ++c.inst_visits;
         lo = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
         ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
         linv = f3{1.f / ld.x, 1.f / ld.y, 1.f / ld.z};
@@ -230,7 +229,7 @@ struct Walker {
       }
       const uint32_t ni = te;
       const float4 na = S.tnodes[3 * ni], nb = S.tnodes[3 * ni + 1];
-      ++c.node_tests;
+      if (kCount) ++c.node_tests;
       // sign-selected slab (geometry/aabb.h:59-71): near/far by direction sign
       const bool sx = d.x < 0.f, sy = d.y < 0.f, sz = d.z < 0.f;
       const float bx0 = sx ? na.w : na.x, bx1 = sx ? na.x : na.w;
@@ -246,7 +245,7 @@ struct Walker {
       if (count > 0) {
         for (uint32_t k = count; k-- > 0;) st.push(kInstTag | S.torder[child + k]);
       } else {
-        ++c.interior;
+        if (kCount) ++c.interior;
         const float4 nc = S.tnodes[3 * ni + 2];
         const int ax = static_cast<int>(__float_as_uint(nc.z));
         const float oa = comp(o, ax), ia = comp(inv, ax);
@@ -276,8 +275,10 @@ struct Walker {
     const float4 la = S.bnodes[2 * child], lb = S.bnodes[2 * child + 1];
     const float4 ra = S.bnodes[2 * child + 2], rb = S.bnodes[2 * child + 3];
     float tn0, tn1;
-    ++c.interior;
-    c.node_tests += 2;
+    if (kCount) {
+      ++c.interior;
+      c.node_tests += 2;
+    }
     const int m = pair_test(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1);
     const uint32_t el = blas_entry(child, __float_as_uint(lb.z), __float_as_uint(lb.w));
     const uint32_t er = blas_entry(child + 1, __float_as_uint(rb.z), __float_as_uint(rb.w));
@@ -309,14 +310,13 @@ struct Walker {
     }
     for (uint32_t k = 0; k < count; ++k) {
       float t, u, v;
-      ++c.tri_tests;
+      if (kCount) ++c.tri_tests;
       if (tri_test(S.tris, first + k, lo, ld, tmin, tbest, t, u, v)) {
         if (kAny) {
           found = true;
           return true;
         }
-        tbest = t;
-        hit.t = t;
+        tbest = t;   // the hit distance (hit.t is not kept during the walk)
         hit.u = u;
         hit.v = v;
         hit.slot = first + k;
@@ -336,13 +336,9 @@ struct Walker {
   // phases are wave-wide over those lanes.
   __device__ __forceinline__ bool round(const DevScene& S, TravStack& st, TravCount& c) {
     if (tlas_steps(S, st, c)) return true;
-    for (uint32_t guard = 0;; ++guard) {
+    for (;;) {
       const bool walk = leaf == 0u && ent_interior(e);
       if (__ballot(walk) == 0ull) break;
-      if (guard >= kPhaseStepLimit) {
-        stuck = true;
-        return true;
-      }
       if (walk) interior_step(S, st, c);
       if (st.overflow) break;
     }
@@ -350,13 +346,9 @@ struct Walker {
       end_instance();
       return true;
     }
-    for (uint32_t guard = 0;; ++guard) {
+    for (;;) {
       const bool has = leaf != 0u;
       if (__ballot(has) == 0ull) break;
-      if (guard >= kPhaseStepLimit) {
-        stuck = true;
-        return true;
-      }
       if (has && leaf_step(S, st, c)) return true;
     }
     if (e == kEntEnd && leaf == 0u) end_instance();
@@ -364,8 +356,8 @@ struct Walker {
   }
 };
 
-// Whole walk of one ray per lane (the ray-batch API). Every lane of the
-// wave must reach it.
+// Whole walk of one ray per lane (the ray-batch API), with work counts.
+// Every lane of the wave must reach it.
 template <bool kAny>
 __device__ __forceinline__ bool walk_scene(const DevScene& S, f3 o, f3 d, float tmin, float tmax, bool valid,
                                            HitRef& hit, TravStack& st, TravCount& c) {
@@ -381,6 +373,7 @@ __device__ __forceinline__ bool walk_scene(const DevScene& S, f3 o, f3 d, float 
   }
   if (w.stuck) st.overflow = true;
   hit = w.hit;
+  hit.t = w.found ? w.tbest : 0.f;
   return valid && w.found;
 }
 

@@ -36,18 +36,21 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
-// Wave-reduced traversal counters; `any` selects the shadow slots.
-__device__ __forceinline__ void flush_counts(unsigned long long* stats, const TravCount& c, uint32_t rays, bool any) {
+// Wave-reduced ray and (with `work`) traversal counters; `any` selects the
+// shadow slots.
+__device__ __forceinline__ void flush_counts(unsigned long long* stats, const TravCount& c, uint32_t rays, bool any,
+                                             bool work = true) {
+  const unsigned long long r = wave_sum(rays);
+  if (lane_id() == 0 && r) atomicAdd(&stats[any ? kStShadow : kStClosest], r);
+  if (!work) return;
   const unsigned long long a = wave_sum(c.node_tests), b = wave_sum(c.interior), t = wave_sum(c.tri_tests),
                            i = wave_sum(c.inst_visits);
-  const unsigned long long r = wave_sum(rays);
   const int base = any ? kStShNodeTests : kStNodeTests;
   if (lane_id() == 0) {
     if (a) atomicAdd(&stats[base], a);
     if (b) atomicAdd(&stats[base + 1], b);
     if (t) atomicAdd(&stats[base + 2], t);
     if (i) atomicAdd(&stats[base + 3], i);
-    if (r) atomicAdd(&stats[any ? kStShadow : kStClosest], r);
   }
 }
 

@@ -265,22 +265,25 @@ __global__ void __launch_bounds__(256) k_tile_weights(const uint16_t* w, uint16_
 // Forward + loss + backward over n training samples; accumulates the
 // gradients of sum_i l_i into grid_g (L x T x 2) and w_g (kWTotal), and the
 // summed loss into loss_sum.
-// With n_dev non-null the sample count is min(*n_dev, n) and lam_scale is
-// taken as count / *lam_sum_dev (the batch mean of lambda scaled to 1).
+// With n_dev non-null the record count is cnt = min(*n_dev, n); the kernel
+// trains on slice `slice` of `nslices` equal parts of [0, cnt) and takes
+// lam_scale = cnt / *lam_sum_dev (the batch mean of lambda scaled to 1).
 __global__ void __launch_bounds__(64) k_guide_train(GuideNet net, int n, const float* pos, const float* nrm,
                                                     const float* dir, const float* lam, float lam_scale,
-                                                    const uint32_t* n_dev, const float* lam_sum_dev,
-                                                    float* grid_g, float* w_g, float* loss_sum) {
+                                                    const uint32_t* n_dev, const float* lam_sum_dev, int slice,
+                                                    int nslices, float* grid_g, float* w_g, float* loss_sum) {
   __shared__ TrainLds s;
+  int begin = 0;
   if (n_dev) {
     const uint32_t cnt = min(*n_dev, static_cast<uint32_t>(n));
     const float sum = *lam_sum_dev;
-    n = static_cast<int>(cnt);
     lam_scale = sum > 0.f ? static_cast<float>(cnt) / sum : 0.f;
+    begin = static_cast<int>((static_cast<uint64_t>(cnt) * slice) / nslices);
+    n = static_cast<int>((static_cast<uint64_t>(cnt) * (slice + 1)) / nslices);
   }
-  if (static_cast<int>(blockIdx.x) * 64 >= n) return;
+  if (begin + static_cast<int>(blockIdx.x) * 64 >= n) return;
   const int lane = glane();
-  const int i = static_cast<int>(blockIdx.x) * 64 + lane;
+  const int i = begin + static_cast<int>(blockIdx.x) * 64 + lane;
   const bool ok = i < n;
   const f3 p = ok ? f3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]} : f3{net.lo[0], net.lo[1], net.lo[2]};
   const f3 nn = ok ? f3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]} : f3{0.f, 0.f, 1.f};
@@ -328,16 +331,18 @@ __global__ void __launch_bounds__(64) k_guide_train(GuideNet net, int n, const f
 
 // Adam with bias correction on FP32 masters; refreshes the FP16 copies and
 // clears the gradients. grad_scale converts summed gradients to means.
-// With n_dev non-null the gradient scale is 1 / *n_dev (no step when zero).
+// With n_dev non-null the gradient scale is nslices / min(*n_dev, cap), one
+// over the size of a training slice (no step when there are no records).
 __global__ void __launch_bounds__(256) k_guide_adam(float* master, uint16_t* half_bits, float* grad, float* m,
                                                     float* v, size_t count, float lr, float b1, float b2, float eps,
-                                                    float bc1, float bc2, float grad_scale, const uint32_t* n_dev) {
+                                                    float bc1, float bc2, float grad_scale, const uint32_t* n_dev,
+                                                    uint32_t cap, int nslices) {
   const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= count) return;
   if (n_dev) {
-    const uint32_t cnt = *n_dev;
-    if (cnt == 0) return;
-    grad_scale = 1.f / static_cast<float>(cnt);
+    const uint32_t cnt = min(*n_dev, cap);
+    if (cnt < static_cast<uint32_t>(nslices)) return;
+    grad_scale = static_cast<float>(nslices) / static_cast<float>(cnt);
   }
   const float g = grad[i] * grad_scale;
   const float mi = b1 * m[i] + (1.f - b1) * g;
@@ -371,19 +376,21 @@ hipError_t launch_guide_query(hipStream_t s, const GuideNet& net, int n, const f
 
 hipError_t launch_guide_train(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
                               const float* dir, const float* lam, float lam_scale, const uint32_t* n_dev,
-                              const float* lam_sum_dev, float* grid_g, float* w_g, float* loss_sum) {
+                              const float* lam_sum_dev, int slice, int nslices, float* grid_g, float* w_g,
+                              float* loss_sum) {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_guide_train, dim3((n + 63) / 64), dim3(64), 0, s, net, n, pos, nrm, dir, lam, lam_scale, n_dev,
-                     lam_sum_dev, grid_g, w_g, loss_sum);
+  const int per = n_dev ? (n + nslices - 1) / nslices + 1 : n;
+  hipLaunchKernelGGL(k_guide_train, dim3((per + 63) / 64), dim3(64), 0, s, net, n, pos, nrm, dir, lam, lam_scale,
+                     n_dev, lam_sum_dev, slice, nslices, grid_g, w_g, loss_sum);
   return hipGetLastError();
 }
 
 hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, float* grad, float* m, float* v,
                              size_t count, float lr, float b1, float b2, float eps, float bc1, float bc2,
-                             float grad_scale, const uint32_t* n_dev) {
+                             float grad_scale, const uint32_t* n_dev, uint32_t cap, int nslices) {
   const size_t blocks = (count + 255) / 256;
   hipLaunchKernelGGL(k_guide_adam, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, master, half_copy, grad, m,
-                     v, count, lr, b1, b2, eps, bc1, bc2, grad_scale, n_dev);
+                     v, count, lr, b1, b2, eps, bc1, bc2, grad_scale, n_dev, cap, nslices);
   return hipGetLastError();
 }
 

@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
 // seg: queue entries per chunk sample slot (1 path queue, 2 shadow queue).
 // Must be reached by every lane of the wave.
 // ---------------------------------------------------------------------------
-template <bool kAny, class Load, class Finish>
+template <bool kAny, bool kCount, class Load, class Finish>
 __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchView& v, uint32_t* cursor,
                                             uint32_t seg, TravStack& st, TravCount& c, uint32_t& rays,
                                             bool& overflow, Load load, Finish finish) {
@@ -106,7 +106,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
   const int lane = lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   uint32_t my = kIdle;
-  Walker<kAny> w;
+  Walker<kAny, kCount> w;
   uint32_t pnext = 0, pend = 0;   // wave-uniform pool of offsets in chunk pci
   int pci = 0;
   int held = -1;                  // fetched unit of another job, waiting for the wave to drain
@@ -170,8 +170,10 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
 }
 
 // ---------------------------------------------------------------------------
-// closest: persistent closest-hit traversal of queue qi.
+// closest: persistent closest-hit traversal of queue qi (kCount: with the
+// traversal work counters).
 // ---------------------------------------------------------------------------
+template <bool kCount>
 __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int cursor, int lds_depth) {
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
@@ -185,7 +187,7 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
-  trace_queue<false>(
+  trace_queue<false, kCount>(
       B, v, &B.counters[kCursor + cursor], 1u, st, c, rays, overflow,
       [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
         const float4 r0 = q.ray0[i], r1 = q.ray1[i];
@@ -194,11 +196,11 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
         tmin = kBounceOffset;
         tmax = __builtin_huge_valf();
       },
-      [&](uint32_t i, const Walker<false>& w) {
-        B.hit[i] = make_float4(w.hit.t, w.hit.u, w.hit.v, __int_as_float(w.found ? w.hit.inst : -1));
+      [&](uint32_t i, const Walker<false, kCount>& w) {
+        B.hit[i] = make_float4(w.found ? w.tbest : 0.f, w.hit.u, w.hit.v, __int_as_float(w.found ? w.hit.inst : -1));
         B.hit_slot[i] = w.hit.slot;
       });
-  flush_counts(B.stats, c, rays, false);
+  flush_counts(B.stats, c, rays, false, kCount);
   if (overflow) atomicAdd(&B.counters[kOverflow], 1u);
 }
 
@@ -282,7 +284,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
           const bool delta = m.delta();
           // guided vertex: one-sample MIS of the stored guide mixture and
           // the BSDF sampler, with probabilities alpha and 1 - alpha
-          const float alpha = B.guide_alpha;
+          const float alpha = kGuided ? *B.alpha_now : 0.f;
           const bool guided = kGuided && lambert_exact(m);
           // area-light NEE (cpu_tiled.cpp:88-139)
           if (S.num_lights > 0 && !delta) {
@@ -463,6 +465,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
 // shadow: persistent any-hit traversal; unoccluded contributions are added
 // to their path's per-slot NEE sum (one writer per (slot, light kind)).
 // ---------------------------------------------------------------------------
+template <bool kCount>
 __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int lds_depth) {
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + kShCnt;
@@ -471,7 +474,7 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
-  trace_queue<true>(
+  trace_queue<true, kCount>(
       B, v, &B.counters[kCursor + cursor], 2u, st, c, rays, overflow,
       [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
         const float4 r0 = B.sh0[i], r1 = B.sh1[i];
@@ -480,7 +483,7 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
         tmin = kShadowOff;
         tmax = r1.z;
       },
-      [&](uint32_t i, const Walker<true>& w) {
+      [&](uint32_t i, const Walker<true, kCount>& w) {
         if (!w.found) {
           const uint32_t k = __float_as_uint(B.sh1[i].w);
           const float4 cv = B.sh_c[i];
@@ -491,7 +494,7 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
           B.l_nee[k] = acc;
         }
       });
-  flush_counts(B.stats, c, rays, true);
+  flush_counts(B.stats, c, rays, true, kCount);
   if (overflow) atomicAdd(&B.counters[kOverflow], 1u);
 }
 
@@ -623,7 +626,7 @@ int trace_grid(int block, int lds_depth) {
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
   int per_cu = 0;
   const size_t lds = static_cast<size_t>(block) * lds_depth * sizeof(uint32_t);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_closest), block, lds) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_closest<false>), block, lds) !=
       hipSuccess)
     return -1;
   if (per_cu < 1) per_cu = 1;
@@ -645,9 +648,10 @@ hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths) {
 }
 
 hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block,
-                          int lds_depth) {
-  hipLaunchKernelGGL(k_closest, dim3(grid), dim3(block), static_cast<size_t>(block) * lds_depth * 4, s, B, qi, cursor,
-                     lds_depth);
+                          int lds_depth, bool count) {
+  const size_t lds = static_cast<size_t>(block) * lds_depth * 4;
+  if (count) hipLaunchKernelGGL(k_closest<true>, dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+  else hipLaunchKernelGGL(k_closest<false>, dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
   return hipGetLastError();
 }
 
@@ -657,9 +661,11 @@ hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, 
   return hipGetLastError();
 }
 
-hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth) {
-  hipLaunchKernelGGL(k_shadow, dim3(grid), dim3(block), static_cast<size_t>(block) * lds_depth * 4, s, B, cursor,
-                     lds_depth);
+hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth,
+                         bool count) {
+  const size_t lds = static_cast<size_t>(block) * lds_depth * 4;
+  if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(block), lds, s, B, cursor, lds_depth);
+  else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(block), lds, s, B, cursor, lds_depth);
   return hipGetLastError();
 }
 

@@ -13,9 +13,10 @@ int num_cus();
 
 hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths);
 hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block,
-                          int lds_depth);
+                          int lds_depth, bool count);
 hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, int grid);
-hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth);
+hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth,
+                         bool count);
 hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int max_pixels);
 hipError_t launch_advance(hipStream_t s, uint32_t* counters);
 hipError_t launch_average(hipStream_t s, const float4* accum, float* out, int npix, float inv_spp);
@@ -30,10 +31,11 @@ hipError_t launch_guide_query(hipStream_t s, const GuideNet& net, int n, const f
                               const float* u, float* dir, float* pdf);
 hipError_t launch_guide_train(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
                               const float* dir, const float* lam, float lam_scale, const uint32_t* n_dev,
-                              const float* lam_sum_dev, float* grid_g, float* w_g, float* loss_sum);
+                              const float* lam_sum_dev, int slice, int nslices, float* grid_g, float* w_g,
+                              float* loss_sum);
 hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, float* grad, float* m, float* v,
                              size_t count, float lr, float b1, float b2, float eps, float bc1, float bc2,
-                             float grad_scale, const uint32_t* n_dev);
+                             float grad_scale, const uint32_t* n_dev, uint32_t cap, int nslices);
 hipError_t launch_tile_weights(hipStream_t s, const uint16_t* w, uint16_t* wt);
 hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, const GuideNet& net, int grid);
 hipError_t launch_guide_collect(hipStream_t s, const WaveBuffers& B, float* pos, float* nrm, float* dir, float* lam,

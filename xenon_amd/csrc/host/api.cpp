@@ -62,6 +62,8 @@ struct GuideState {
   float4* rec = nullptr;
   float *t_pos = nullptr, *t_nrm = nullptr, *t_dir = nullptr, *t_lam = nullptr, *t_lam_sum = nullptr;
   uint32_t* t_n = nullptr;
+  float* alpha_dev = nullptr;     // MIS probability of the guide for the next batch
+  float alpha_host = 0.f;
   bool train = true;
 };
 
@@ -72,7 +74,8 @@ struct xnh_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
   std::vector<hipEvent_t> prof_events;
-  bool profiling = false;
+  bool profiling = false;         // per-kernel HIP-event timing
+  bool counting = false;          // traversal work counters (the counting kernel variants)
   bool use_graph = true;
   xnh_gpu_scene own;              // the scene of xnh_upload_scene
   bool has_scene = false;
@@ -459,14 +462,14 @@ int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
   for (int d = 0; d < depth; ++d) {
     const int qi = d & 1;
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
-    HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth));
+    HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth, c->counting));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     if (c->guide.on) HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, c->guide.net, c->grid_shade * 2));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     HIP_OR_FAIL(c, launch_shade(s, c->B, qi, d, c->grid_shade));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
-    HIP_OR_FAIL(c, launch_shadow(s, c->B, 2 * d + 1, c->grid_trace, c->block, c->lds_depth));
+    HIP_OR_FAIL(c, launch_shadow(s, c->B, 2 * d + 1, c->grid_trace, c->block, c->lds_depth, c->counting));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
   }
   HIP_OR_FAIL(c, launch_accumulate(s, c->B, cap));
@@ -500,27 +503,39 @@ int ensure_graph(xnh_ctx* c, int depth) {
   return XNH_OK;
 }
 
-// One guiding step on the batch just rendered: records -> training batch
-// (all counts stay on the device) -> fused forward/backward -> Adam.
+// Guiding steps on the batch just rendered: records -> training batch (all
+// counts stay on the device) -> `train_steps` fused forward/backward + Adam
+// steps on equal slices of it; then the guide's MIS probability for the
+// next batch (ramped over the warm-up steps).
 int guide_train_batch(xnh_ctx* c) {
   GuideState& g = c->guide;
   hipStream_t s = c->stream;
   const int R = static_cast<int>(c->B.rec_cap);
+  const int K = g.cfg.train_steps;
   HIP_OR_FAIL(c, hipMemsetAsync(g.t_lam_sum, 0, sizeof(float), s));
   HIP_OR_FAIL(c, launch_guide_collect(s, c->B, g.t_pos, g.t_nrm, g.t_dir, g.t_lam, g.t_lam_sum, g.t_n));
-  HIP_OR_FAIL(c, hipMemsetAsync(g.loss, 0, sizeof(float), s));
-  HIP_OR_FAIL(c, launch_guide_train(s, g.net, R, g.t_pos, g.t_nrm, g.t_dir, g.t_lam, 1.f, g.t_n, g.t_lam_sum, g.grid_g,
-                                    g.w_g, g.loss));
-  g.step++;
   const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
-  const float bc1 = 1.f - static_cast<float>(std::pow(double(b1), g.step));
-  const float bc2 = 1.f - static_cast<float>(std::pow(double(b2), g.step));
   const float lr = g.cfg.learning_rate;
-  HIP_OR_FAIL(c, launch_guide_adam(s, g.grid_m, g.grid_h, g.grid_g, g.grid_am, g.grid_av, g.ngrid, lr, b1, b2, eps, bc1,
-                                   bc2, 1.f, g.t_n));
-  HIP_OR_FAIL(c, launch_guide_adam(s, g.w_m, g.w_h, g.w_g, g.w_am, g.w_av, size_t(kWTotal), lr, b1, b2, eps, bc1, bc2,
-                                   1.f, g.t_n));
-  HIP_OR_FAIL(c, launch_tile_weights(s, g.w_h, g.w_t));
+  for (int k = 0; k < K; ++k) {
+    HIP_OR_FAIL(c, hipMemsetAsync(g.loss, 0, sizeof(float), s));
+    HIP_OR_FAIL(c, launch_guide_train(s, g.net, R, g.t_pos, g.t_nrm, g.t_dir, g.t_lam, 1.f, g.t_n, g.t_lam_sum, k, K,
+                                      g.grid_g, g.w_g, g.loss));
+    g.step++;
+    const float bc1 = 1.f - static_cast<float>(std::pow(double(b1), g.step));
+    const float bc2 = 1.f - static_cast<float>(std::pow(double(b2), g.step));
+    HIP_OR_FAIL(c, launch_guide_adam(s, g.grid_m, g.grid_h, g.grid_g, g.grid_am, g.grid_av, g.ngrid, lr, b1, b2, eps,
+                                     bc1, bc2, 1.f, g.t_n, static_cast<uint32_t>(R), K));
+    HIP_OR_FAIL(c, launch_guide_adam(s, g.w_m, g.w_h, g.w_g, g.w_am, g.w_av, size_t(kWTotal), lr, b1, b2, eps, bc1,
+                                     bc2, 1.f, g.t_n, static_cast<uint32_t>(R), K));
+    HIP_OR_FAIL(c, launch_tile_weights(s, g.w_h, g.w_t));
+  }
+  const float a = g.cfg.warmup_steps > 0
+                      ? g.cfg.guide_fraction * std::min(1.f, static_cast<float>(g.step) / g.cfg.warmup_steps)
+                      : g.cfg.guide_fraction;
+  if (a != g.alpha_host) {
+    g.alpha_host = a;
+    HIP_OR_FAIL(c, hipMemcpyAsync(g.alpha_dev, &g.alpha_host, sizeof(float), hipMemcpyHostToDevice, s));
+  }
   return XNH_OK;
 }
 
@@ -767,9 +782,14 @@ void xnh_scene_release(xnh_ctx* c, xnh_gpu_scene* g) {
   delete g;
 }
 
-int xnh_set_profiling(xnh_ctx* c, int32_t on) {
-  if (!c) return XNH_E_INVALID;
-  c->profiling = on != 0;
+int xnh_set_profiling(xnh_ctx* c, int32_t flags) {
+  if (!c || (flags & ~(XNH_PROFILE_TIMING | XNH_PROFILE_WORK)) != 0) return XNH_E_INVALID;
+  c->profiling = (flags & XNH_PROFILE_TIMING) != 0;
+  const bool work = (flags & XNH_PROFILE_WORK) != 0;
+  if (work != c->counting) {
+    c->counting = work;
+    c->table_gen++;   // the captured graph holds the other kernel variants
+  }
   return XNH_OK;
 }
 
@@ -905,6 +925,7 @@ int xnh_set_tuning(xnh_ctx* c, int32_t block_size, int32_t lds_stack, int32_t sp
   c->B.rec_cap = keep.rec_cap;
   c->B.rec_every = keep.rec_every;
   c->B.guide_alpha = keep.guide_alpha;
+  c->B.alpha_now = keep.alpha_now;
   c->spp_total = 0;
   c->table_gen++;   // the captured graph holds the old buffers
   return XNH_OK;
@@ -993,6 +1014,7 @@ int xnh_guide_disable(xnh_ctx* c) {
   c->B.rec_cap = 0;
   c->B.rec_every = 0;
   c->B.guide_alpha = 0.f;
+  c->B.alpha_now = nullptr;
   c->table_gen++;
   return XNH_OK;
 }
@@ -1011,6 +1033,8 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (cfg.guide_fraction <= 0.f) cfg.guide_fraction = 0.5f;
   if (cfg.train_batch <= 0) cfg.train_batch = 1 << 18;
   if (cfg.record_paths <= 0) cfg.record_paths = 1 << 16;
+  if (cfg.train_steps <= 0) cfg.train_steps = 1;
+  if (cfg.warmup_steps < 0) cfg.warmup_steps = 0;
   if (cfg.t_log2 < 8 || cfg.t_log2 > 24 || cfg.guide_fraction >= 1.f) return fail(c, XNH_E_INVALID, "guide config");
   xnh_guide_disable(c);
   GuideState& g = c->guide;
@@ -1039,6 +1063,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_lam, R);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_lam_sum, size_t(1));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_n, size_t(1));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.alpha_dev, size_t(1));
   if (rc != XNH_OK) {
     xnh_guide_disable(c);
     return rc;
@@ -1051,6 +1076,12 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   c->B.rec_cap = static_cast<uint32_t>(R);
   c->B.rec_every = static_cast<uint32_t>(std::max<size_t>(1, cap / static_cast<size_t>(cfg.record_paths)));
   c->B.guide_alpha = cfg.guide_fraction;
+  c->B.alpha_now = g.alpha_dev;
+  g.alpha_host = cfg.warmup_steps > 0 ? 0.f : cfg.guide_fraction;
+  if (hipMemcpy(g.alpha_dev, &g.alpha_host, sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    xnh_guide_disable(c);
+    return fail(c, XNH_E_HIP, "guide alpha upload");
+  }
   c->table_gen++;   // the captured batch graph gains the guide kernels
   g.net.t_log2 = cfg.t_log2;
   const double b = std::exp((std::log(2048.0) - std::log(16.0)) / (kGLevels - 1));
@@ -1157,7 +1188,8 @@ static int guide_step(xnh_ctx* c, int n, const float* dp, const float* dn, const
   GuideState& g = c->guide;
   hipStream_t s = c->stream;
   HIP_OR_FAIL(c, hipMemsetAsync(g.loss, 0, 4, s));
-  HIP_OR_FAIL(c, launch_guide_train(s, g.net, n, dp, dn, dd, dl, lam_scale, nullptr, nullptr, g.grid_g, g.w_g, g.loss));
+  HIP_OR_FAIL(c, launch_guide_train(s, g.net, n, dp, dn, dd, dl, lam_scale, nullptr, nullptr, 0, 1, g.grid_g, g.w_g,
+                                    g.loss));
   if (apply) {
     g.step++;
     const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
@@ -1165,9 +1197,9 @@ static int guide_step(xnh_ctx* c, int n, const float* dp, const float* dn, const
     const float bc2 = 1.f - static_cast<float>(std::pow(double(b2), g.step));
     const float lr = g.cfg.learning_rate, gs = 1.f / static_cast<float>(n);
     HIP_OR_FAIL(c, launch_guide_adam(s, g.grid_m, g.grid_h, g.grid_g, g.grid_am, g.grid_av, g.ngrid, lr, b1, b2, eps,
-                                     bc1, bc2, gs, nullptr));
+                                     bc1, bc2, gs, nullptr, 0u, 1));
     HIP_OR_FAIL(c, launch_guide_adam(s, g.w_m, g.w_h, g.w_g, g.w_am, g.w_av, size_t(kWTotal), lr, b1, b2, eps, bc1,
-                                     bc2, gs, nullptr));
+                                     bc2, gs, nullptr, 0u, 1));
     HIP_OR_FAIL(c, launch_tile_weights(s, g.w_h, g.w_t));
   }
   return XNH_OK;

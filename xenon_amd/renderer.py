@@ -210,8 +210,13 @@ class HipRenderer:
     def reset(self):
         self._chk(N.lib().xnh_reset(self.h))
 
-    def set_profiling(self, on: bool):
-        self._chk(N.lib().xnh_set_profiling(self.h, 1 if on else 0))
+    PROFILE_TIMING, PROFILE_WORK = 1, 2
+
+    def set_profiling(self, timing: bool = False, work: bool = False):
+        """Per-kernel event timing and/or traversal work counters for the
+        following renders (both off by default)."""
+        self._chk(N.lib().xnh_set_profiling(self.h, (self.PROFILE_TIMING if timing else 0) |
+                                            (self.PROFILE_WORK if work else 0)))
 
     def stats(self) -> dict:
         s = N.Stats()
@@ -296,7 +301,8 @@ class HipRenderer:
     GUIDE_LEVELS, GUIDE_FEATURES, GUIDE_WEIGHTS = 16, 2, 48 * 64 + 64 * 64 + 64 * 64 + 64 * 48
 
     def guide_enable(self, bounds_lo=None, bounds_hi=None, t_log2: int = 19, learning_rate: float = 5e-3,
-                     guide_fraction: float = 0.5, seed: int = 0, train_batch: int = 0, record_paths: int = 0):
+                     guide_fraction: float = 0.5, seed: int = 0, train_batch: int = 0, record_paths: int = 0,
+                     train_steps: int = 1, warmup_steps: int = 0):
         """Turns learned path guiding on with fresh parameters; positions
         inside [bounds_lo, bounds_hi] (default: the uploaded scene's box)
         are mapped to the hash grid."""
@@ -306,7 +312,7 @@ class HipRenderer:
             bounds_lo, bounds_hi = self.scene.bounds()
         cfg = N.GuideConfig(int(t_log2), float(learning_rate), float(guide_fraction), int(seed) & 0xFFFFFFFF,
                             (C.c_float * 3)(*bounds_lo), (C.c_float * 3)(*bounds_hi), int(train_batch),
-                            int(record_paths))
+                            int(record_paths), int(train_steps), int(warmup_steps))
         self._chk(N.lib().xnh_guide_enable(self.h, C.byref(cfg)))
         self._guide_t = 1 << int(t_log2 or 19)
 
