@@ -131,6 +131,7 @@ struct WaveBuffers {
   uint32_t rec_every;   // a path records its vertices when hash(slot, batch) % rec_every == 0
   float guide_alpha;    // largest one-sample MIS probability of the guide; 0 = guiding off
   const float* alpha_now;   // the probability used by the current batch (device memory)
+  int32_t guide_depth;      // bounces below this depth are guided
 };
 
 enum Ctr : int {

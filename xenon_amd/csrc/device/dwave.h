@@ -107,6 +107,28 @@ __device__ __forceinline__ Surface resolve(const DevScene& S, const HitRef& h, f
   return s;
 }
 
+// The part of `resolve` the guide needs: position, world geometric normal,
+// material and the light-map lookup of the hit triangle.
+struct GeoSurface {
+  f3 pos, gn;
+  int mat;
+  uint32_t tri;
+  int32_t lmap;
+};
+
+__device__ __forceinline__ GeoSurface resolve_geo(const DevScene& S, const HitRef& h, f3 o, f3 d) {
+  const InstRec in = load_inst(S.inst, h.inst);
+  const float4 a = S.tris[3 * h.slot], b = S.tris[3 * h.slot + 1], c = S.tris[3 * h.slot + 2];
+  const f3 e1{a.w, b.x, b.y}, e2{b.z, b.w, c.x};
+  GeoSurface s;
+  s.tri = __float_as_uint(c.y);
+  s.pos = o + h.t * d;
+  s.gn = normal_out(in, normalize(cross(e1, e2)));
+  s.mat = S.tmat[(in.tri_base & ~kFlatBit) + s.tri];
+  s.lmap = in.lmap;
+  return s;
+}
+
 // Gradient sky (scene/environment.h:34-46).
 __device__ __forceinline__ f3 sky_eval(const DevSky& k, f3 d) {
   const f3 hz{k.horizon[0], k.horizon[1], k.horizon[2]};

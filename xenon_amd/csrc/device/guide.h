@@ -182,8 +182,9 @@ __device__ __forceinline__ void dense_reg(const h4v* x, const h4v* w, h4v* y) {
 }
 
 // wave: forward pass of 64 points, one per lane (p: world position, n:
-// normal). lds: 64 x kGIn FP16 scratch of this wave. On return the lane's
-// 48 raw outputs (FP32 accumulation, stored FP16) are in lds[lane * kGIn ..].
+// normal). lds: 2 x 64 x kGIn FP16 scratch of this wave. On return the
+// lane's 48 raw outputs (FP32 accumulation, stored FP16) are at
+// lds + 64 kGIn + lane kGIn.
 __device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3 n, _Float16* lds) {
   const int lane = static_cast<int>(threadIdx.x & 63u), g = lane >> 4, li = lane & 15;
   guide_encode(net, p, n, lds + lane * kGIn);
@@ -192,7 +193,7 @@ __device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3
   for (int nt = 0; nt < 4; ++nt)
     for (int kt = 0; kt < 3; ++kt)
       x[nt][kt] = *reinterpret_cast<const h4v*>(lds + (16 * nt + li) * kGIn + 16 * kt + 4 * g);
-  __syncthreads();
+  _Float16* out = lds + 64 * kGIn;
   const h4v* wt = reinterpret_cast<const h4v*>(net.wt);
   for (int nt = 0; nt < 4; ++nt) {
     h4v a[4], b[4];
@@ -203,7 +204,7 @@ __device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3
       f4v acc = {0.f, 0.f, 0.f, 0.f};
       for (int kt = 0; kt < 4; ++kt)
         acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wt[kW4 / 4 + (mt * 4 + kt) * 64 + lane], a[kt], acc, 0, 0, 0);
-      for (int j = 0; j < 4; ++j) lds[(16 * nt + li) * kGIn + 16 * mt + 4 * g + j] = static_cast<_Float16>(acc[j]);
+      for (int j = 0; j < 4; ++j) out[(16 * nt + li) * kGIn + 16 * mt + 4 * g + j] = static_cast<_Float16>(acc[j]);
     }
   }
   __syncthreads();
@@ -268,16 +269,32 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u
 //   k = 10, 11 pi_j * C(kappa_j), the pdf weight of lobe j
 // (kMixVecs = 12, guide_net.h)
 
-__device__ __forceinline__ void store_mixture(float4* mix, uint32_t C, uint32_t i, const Mixture& m) {
-  float w[kGLobes];
+// Decodes raw outputs (FP16, 5 per lobe) straight into the stored form,
+// with decode_mixture's arithmetic, four values at a time.
+__device__ __forceinline__ float raw_kappa(const _Float16* raw, int j) {
+  return expf(fminf(fmaxf(static_cast<float>(raw[5 * j + 3]), kKappaLogMin), kKappaLogMax));
+}
+__device__ __forceinline__ void decode_store_mixture(const _Float16* raw, float4* mix, uint32_t C, uint32_t i) {
+  float amax = -3.0e38f;
+  for (int j = 0; j < kGLobes; ++j) amax = fmaxf(amax, static_cast<float>(raw[5 * j + 4]));
+  float den = 0.f;
+  for (int j = 0; j < kGLobes; ++j) den += expf(static_cast<float>(raw[5 * j + 4]) - amax);
   for (int j = 0; j < kGLobes; ++j) {
-    mix[j * C + i] = make_float4(m.mu[j].x, m.mu[j].y, m.mu[j].z, m.kappa[j]);
-    w[j] = m.pi[j] * vmf_norm(m.kappa[j]);
+    const float r0 = static_cast<float>(raw[5 * j]), r1 = static_cast<float>(raw[5 * j + 1]),
+                r2 = static_cast<float>(raw[5 * j + 2]);
+    const float nm = sqrtf(r0 * r0 + r1 * r1 + r2 * r2 + 1e-12f);
+    mix[j * C + i] = make_float4(r0 / nm, r1 / nm, r2 / nm, raw_kappa(raw, j));
   }
-  mix[8 * C + i] = make_float4(m.pi[0], m.pi[1], m.pi[2], m.pi[3]);
-  mix[9 * C + i] = make_float4(m.pi[4], m.pi[5], m.pi[6], m.pi[7]);
-  mix[10 * C + i] = make_float4(w[0], w[1], w[2], w[3]);
-  mix[11 * C + i] = make_float4(w[4], w[5], w[6], w[7]);
+  for (int h = 0; h < 2; ++h) {
+    float pi[4], w[4];
+    for (int q = 0; q < 4; ++q) {
+      const int j = 4 * h + q;
+      pi[q] = expf(static_cast<float>(raw[5 * j + 4]) - amax) / den;
+      w[q] = pi[q] * vmf_norm(raw_kappa(raw, j));
+    }
+    mix[(8 + h) * C + i] = make_float4(pi[0], pi[1], pi[2], pi[3]);
+    mix[(10 + h) * C + i] = make_float4(w[0], w[1], w[2], w[3]);
+  }
 }
 
 __device__ __forceinline__ float lane4(float4 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : (k == 2 ? v.z : v.w)); }

@@ -115,15 +115,16 @@ struct TrainLds {
 // Evaluates the network for n points: raw outputs (n x 48, FP32).
 __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const float* pos, const float* nrm,
                                                    float* raw) {
-  __shared__ _Float16 lds[64 * kGIn];
+  __shared__ _Float16 lds[2 * 64 * kGIn];
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
   const bool ok = i < n;
   const f3 p = ok ? f3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]} : f3{net.lo[0], net.lo[1], net.lo[2]};
   const f3 nn = ok ? f3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]} : f3{0.f, 0.f, 1.f};
   guide_forward_wave(net, p, nn, lds);
+  const _Float16* out = lds + 64 * kGIn + lane * kGIn;
   if (ok)
-    for (int k = 0; k < kGOut; ++k) raw[static_cast<size_t>(i) * kGOut + k] = static_cast<float>(lds[lane * kGIn + k]);
+    for (int k = 0; k < kGOut; ++k) raw[static_cast<size_t>(i) * kGOut + k] = static_cast<float>(out[k]);
 }
 
 // Mixture queries for n points: with u non-null, draws one direction per
@@ -131,7 +132,7 @@ __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const fl
 // dir; then writes the mixture pdf at dir[i] (drawn or given) to pdf.
 __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const float* pos, const float* nrm,
                                                     const float* u, float* dir, float* pdf) {
-  __shared__ _Float16 lds[64 * kGIn];
+  __shared__ _Float16 lds[2 * 64 * kGIn];
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
   const bool ok = i < n;
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const f
   guide_forward_wave(net, p, nn, lds);
   if (!ok) return;
   float raw[kGOut];
-  for (int k = 0; k < kGLobes * 5; ++k) raw[k] = static_cast<float>(lds[lane * kGIn + k]);
+  for (int k = 0; k < kGLobes * 5; ++k) raw[k] = static_cast<float>(lds[64 * kGIn + lane * kGIn + k]);
   Mixture m;
   decode_mixture(raw, m);
   f3 w;
@@ -160,12 +161,13 @@ __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const f
 // ---------------------------------------------------------------------------
 
 // Evaluates the guide at every closest hit of queue qi that lands on a
-// guided surface (an exact Lambertian that is not an emitter) and stores
-// the mixture for k_shade. One wave per block, grid-stride over the
-// batch's 64-entry units.
+// guided surface (an exact Lambertian that is not an emitter; the network's
+// normal input is the geometric normal) and stores the mixture for k_shade.
+// One wave per block, grid-stride over the batch's 64-entry units. Launched
+// for the bounces that are guided (below the configured guide depth).
 __global__ void __launch_bounds__(64) k_guide_infer(WaveBuffers B, int qi, GuideNet net) {
   __shared__ BatchView v;
-  __shared__ _Float16 lds[64 * kGIn];
+  __shared__ _Float16 lds[2 * 64 * kGIn];
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
   load_batch(v, B, counts, false);
   const uint32_t total = v.prefix[v.nch];
@@ -185,24 +187,18 @@ __global__ void __launch_bounds__(64) k_guide_infer(WaveBuffers B, int qi, Guide
       if (inst >= 0) {
         const float4 r0 = q.ray0[i], r1 = q.ray1[i];
         const HitRef h{hv.x, hv.y, hv.z, B.hit_slot[i], inst};
-        const Surface sf = resolve(S, h, f3{r0.x, r0.y, r0.z}, f3{r0.w, r1.x, r1.y});
+        const GeoSurface sf = resolve_geo(S, h, f3{r0.x, r0.y, r0.z}, f3{r0.w, r1.x, r1.y});
         const bool emitter = sf.lmap >= 0 && S.lmap[sf.lmap + static_cast<int>(sf.tri)] >= 0;
         if (!emitter && lambert_exact(load_mat(S.mats, sf.mat))) {
           want = true;
           p = sf.pos;
-          n = sf.n;
+          n = sf.gn;
         }
       }
     }
     if (__ballot(want) == 0ull) continue;
     guide_forward_wave(net, p, n, lds);
-    if (want) {
-      float raw[kGOut];
-      for (int k = 0; k < kGLobes * 5; ++k) raw[k] = static_cast<float>(lds[glane() * kGIn + k]);
-      Mixture m;
-      decode_mixture(raw, m);
-      store_mixture(B.mix, C, i, m);
-    }
+    if (want) decode_store_mixture(lds + 64 * kGIn + glane() * kGIn, B.mix, C, i);
   }
 }
 

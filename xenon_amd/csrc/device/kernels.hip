@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
           // guided vertex: one-sample MIS of the stored guide mixture and
           // the BSDF sampler, with probabilities alpha and 1 - alpha
           const float alpha = kGuided ? *B.alpha_now : 0.f;
-          const bool guided = kGuided && lambert_exact(m);
+          const bool guided = kGuided && depth < B.guide_depth && lambert_exact(m);
           // area-light NEE (cpu_tiled.cpp:88-139)
           if (S.num_lights > 0 && !delta) {
             const int nl = S.num_lights;
@@ -395,7 +395,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
                     const uint32_t R = B.rec_cap;
                     const float nan = __builtin_nanf("");
                     B.rec[r] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, T.x);
-                    B.rec[R + r] = make_float4(sf.n.x, sf.n.y, sf.n.z, T.y);
+                    B.rec[R + r] = make_float4(sf.gn.x, sf.gn.y, sf.gn.z, T.y);
                     B.rec[2 * R + r] = make_float4(wiw.x, wiw.y, wiw.z, T.z);
                     B.rec[3 * R + r] = make_float4(nan, nan, nan, __uint_as_float(slot));
                     nrec = r;

@@ -464,7 +464,8 @@ int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth, c->counting));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
-    if (c->guide.on) HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, c->guide.net, c->grid_shade * 2));
+    if (c->guide.on && d < c->B.guide_depth)
+      HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, c->guide.net, c->grid_shade * 2));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     HIP_OR_FAIL(c, launch_shade(s, c->B, qi, d, c->grid_shade));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
@@ -926,6 +927,7 @@ int xnh_set_tuning(xnh_ctx* c, int32_t block_size, int32_t lds_stack, int32_t sp
   c->B.rec_every = keep.rec_every;
   c->B.guide_alpha = keep.guide_alpha;
   c->B.alpha_now = keep.alpha_now;
+  c->B.guide_depth = keep.guide_depth;
   c->spp_total = 0;
   c->table_gen++;   // the captured graph holds the old buffers
   return XNH_OK;
@@ -1015,6 +1017,7 @@ int xnh_guide_disable(xnh_ctx* c) {
   c->B.rec_every = 0;
   c->B.guide_alpha = 0.f;
   c->B.alpha_now = nullptr;
+  c->B.guide_depth = 0;
   c->table_gen++;
   return XNH_OK;
 }
@@ -1035,6 +1038,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (cfg.record_paths <= 0) cfg.record_paths = 1 << 16;
   if (cfg.train_steps <= 0) cfg.train_steps = 1;
   if (cfg.warmup_steps < 0) cfg.warmup_steps = 0;
+  if (cfg.guide_depth <= 0) cfg.guide_depth = 3;
   if (cfg.t_log2 < 8 || cfg.t_log2 > 24 || cfg.guide_fraction >= 1.f) return fail(c, XNH_E_INVALID, "guide config");
   xnh_guide_disable(c);
   GuideState& g = c->guide;
@@ -1077,6 +1081,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   c->B.rec_every = static_cast<uint32_t>(std::max<size_t>(1, cap / static_cast<size_t>(cfg.record_paths)));
   c->B.guide_alpha = cfg.guide_fraction;
   c->B.alpha_now = g.alpha_dev;
+  c->B.guide_depth = cfg.guide_depth;
   g.alpha_host = cfg.warmup_steps > 0 ? 0.f : cfg.guide_fraction;
   if (hipMemcpy(g.alpha_dev, &g.alpha_host, sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
     xnh_guide_disable(c);

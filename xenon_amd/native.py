@@ -115,7 +115,7 @@ class GuideConfig(C.Structure):
     _fields_ = [("t_log2", C.c_int32), ("learning_rate", C.c_float), ("guide_fraction", C.c_float),
                 ("seed", C.c_uint32), ("bounds_lo", C.c_float * 3), ("bounds_hi", C.c_float * 3),
                 ("train_batch", C.c_int32), ("record_paths", C.c_int32), ("train_steps", C.c_int32),
-                ("warmup_steps", C.c_int32)]
+                ("warmup_steps", C.c_int32), ("guide_depth", C.c_int32)]
 
 
 class TuneResult(C.Structure):

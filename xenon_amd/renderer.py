@@ -302,7 +302,7 @@ class HipRenderer:
 
     def guide_enable(self, bounds_lo=None, bounds_hi=None, t_log2: int = 19, learning_rate: float = 5e-3,
                      guide_fraction: float = 0.5, seed: int = 0, train_batch: int = 0, record_paths: int = 0,
-                     train_steps: int = 1, warmup_steps: int = 0):
+                     train_steps: int = 1, warmup_steps: int = 0, guide_depth: int = 0):
         """Turns learned path guiding on with fresh parameters; positions
         inside [bounds_lo, bounds_hi] (default: the uploaded scene's box)
         are mapped to the hash grid."""
@@ -312,7 +312,7 @@ class HipRenderer:
             bounds_lo, bounds_hi = self.scene.bounds()
         cfg = N.GuideConfig(int(t_log2), float(learning_rate), float(guide_fraction), int(seed) & 0xFFFFFFFF,
                             (C.c_float * 3)(*bounds_lo), (C.c_float * 3)(*bounds_hi), int(train_batch),
-                            int(record_paths), int(train_steps), int(warmup_steps))
+                            int(record_paths), int(train_steps), int(warmup_steps), int(guide_depth))
         self._chk(N.lib().xnh_guide_enable(self.h, C.byref(cfg)))
         self._guide_t = 1 << int(t_log2 or 19)
 
