@@ -335,6 +335,32 @@ int xnh_scene_flat(const xnh_scene* s, xnh_flat_scene* out);
 int xnh_camera_look_at(const float* eye3, const float* target3, const float* up3, float fov_deg, float aspect,
                        xnh_camera* out);
 
+/* ---- scene files --------------------------------------------------------
+ * load_scene (scene/scene_file.cpp:49-240): a .xenon scene file with its
+ * .mat materials (material/material.cpp:31-152) and OBJ meshes
+ * (geometry/mesh.cpp:42-205) into a built host scene plus the file's camera
+ * and render settings. Relative paths resolve as the reference resolves them:
+ * a matfile name as "materials/<name>" under the working directory, else
+ * beside the scene file; a mesh path as given.
+ * Analytic primitives (sphere, box, disk, quad, plane) are not traced by
+ * this backend: they are counted in skipped_primitives, not loaded. Files
+ * that cannot be opened are counted in missing_files and skipped (a .mat
+ * falls back to the default material), as the reference skips them. */
+typedef struct {
+  int32_t width, height, samples;       /* SceneConfig (scene/scene_file.h:11-19) */
+  int32_t min_bounces, max_bounces;
+  int32_t meshes;                       /* meshes loaded */
+  int32_t skipped_primitives;           /* analytic primitives not loaded */
+  int32_t missing_files;                /* .mat / .obj files that did not open */
+  float eye[3], target[3], up[3], fov;  /* the camera directive, for other aspect ratios */
+} xnh_file_info;
+int xnh_scene_load_file(const char* path, xnh_scene** out, xnh_camera* camera, xnh_file_info* info);
+/* load_material (material/material.h:81-85) without the cache: the 16
+ * material parameters above from a .mat file, defaults for absent keys;
+ * name (NUL-terminated, truncated to name_cap) optional. XNH_E_INVALID when
+ * the file does not open (params16 then holds the defaults). */
+int xnh_material_load(const char* path, float* params16, char* name, int32_t name_cap);
+
 #ifdef __cplusplus
 }
 #endif

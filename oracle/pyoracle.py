@@ -259,3 +259,110 @@ class OScene:
         self.lib.dll.orc_path(self.h, width or self.desc.width, height or self.desc.height, x, y, spp,
                               self.desc.min_bounces, self.desc.max_bounces, out.ctypes.data_as(_fp))
         return out
+
+
+# ---- scene files: the reference's own load_scene (oracle/_ref only) --------
+_FILE_SIGS = {
+    "orc_ref_load_scene_file": (_vp, [C.c_char_p, _ip]),
+    "orc_ref_scene_sizes": (None, [_vp, _ip]),
+    "orc_ref_mesh_size": (None, [_vp, C.c_int, _ip]),
+    "orc_ref_mesh_export": (None, [_vp, C.c_int, _fp, _fp, _ip, _ip]),
+    "orc_ref_material_params": (None, [_vp, C.c_int, _fp]),
+    "orc_ref_lights_export": (None, [_vp, _fp]),
+    "orc_ref_suns_export": (None, [_vp, _fp]),
+    "orc_ref_sky_export": (None, [_vp, _fp]),
+}
+
+
+class RefFileScene:
+    """A scene loaded from a .xenon file by the reference's load_scene
+    (scene/scene_file.cpp:49-240), inside oracle/_ref/libxenon_ref.so."""
+
+    def __init__(self, lib: Lib, path: str):
+        if not lib.is_reference:
+            raise ValueError("scene files load through the reference library only")
+        d = lib.dll
+        for name, (res, args) in _FILE_SIGS.items():
+            fn = getattr(d, name)
+            fn.restype, fn.argtypes = res, args
+        self.lib = lib
+        cfg = np.zeros(5, np.int32)
+        self.h = d.orc_ref_load_scene_file(str(path).encode(), cfg.ctypes.data_as(_ip))
+        if not self.h:
+            raise FileNotFoundError(path)
+        self.config = cfg
+        sz = np.zeros(5, np.int32)
+        d.orc_ref_scene_sizes(self.h, sz.ctypes.data_as(_ip))
+        self.num_meshes, self.num_materials, self.num_lights, self.num_suns, self.num_prims = (int(v) for v in sz)
+
+    def __del__(self):
+        try:
+            self.lib.dll.orc_scene_free(self.h)
+        except Exception:
+            pass
+
+    def mesh(self, i: int):
+        d = self.lib.dll
+        nvt = np.zeros(2, np.int32)
+        d.orc_ref_mesh_size(self.h, i, nvt.ctypes.data_as(_ip))
+        nv, nt = int(nvt[0]), int(nvt[1])
+        pos = np.zeros((nv, 3), np.float32)
+        nrm = np.zeros((nv, 3), np.float32)
+        idx = np.zeros((nt, 3), np.int32)
+        mat = np.zeros(nt, np.int32)
+        d.orc_ref_mesh_export(self.h, i, pos.ctypes.data_as(_fp), nrm.ctypes.data_as(_fp), idx.ctypes.data_as(_ip),
+                              mat.ctypes.data_as(_ip))
+        return pos, nrm, idx, mat
+
+    def material_params(self, i: int) -> np.ndarray:
+        out = np.zeros(16, np.float32)
+        self.lib.dll.orc_ref_material_params(self.h, i, out.ctypes.data_as(_fp))
+        return out
+
+    def material_derived(self, i: int) -> np.ndarray:
+        out = np.zeros(11, np.float32)
+        self.lib.dll.orc_material_derived(self.h, i, out.ctypes.data_as(_fp))
+        return out
+
+    def lights(self) -> np.ndarray:
+        out = np.zeros((self.num_lights, 6), np.float32)
+        self.lib.dll.orc_ref_lights_export(self.h, out.ctypes.data_as(_fp))
+        return out
+
+    def suns(self) -> np.ndarray:
+        out = np.zeros((self.num_suns, 6), np.float32)
+        self.lib.dll.orc_ref_suns_export(self.h, out.ctypes.data_as(_fp))
+        return out
+
+    def sky(self) -> np.ndarray:
+        out = np.zeros(11, np.float32)
+        self.lib.dll.orc_ref_sky_export(self.h, out.ctypes.data_as(_fp))
+        return out
+
+    def camera(self) -> np.ndarray:
+        out = np.zeros(12, np.float32)
+        self.lib.dll.orc_camera_get(self.h, out.ctypes.data_as(_fp))
+        return out
+
+    def blas(self, mesh: int):
+        d = self.lib.dll
+        n = d.orc_blas_node_count(self.h, mesh)
+        nvt = np.zeros(2, np.int32)
+        d.orc_ref_mesh_size(self.h, mesh, nvt.ctypes.data_as(_ip))
+        nodes = np.zeros(n, NODE_DTYPE)
+        slots = np.zeros(int(nvt[1]), np.uint32)
+        d.orc_blas_export(self.h, mesh, nodes.ctypes.data_as(_vp), slots.ctypes.data_as(_u32p))
+        return nodes, slots
+
+    def tlas(self):
+        d = self.lib.dll
+        n = d.orc_tlas_node_count(self.h)
+        ni = d.orc_instance_count(self.h)
+        nodes = np.zeros(n, NODE_DTYPE)
+        order = np.zeros(ni, np.uint32)
+        d.orc_tlas_export(self.h, nodes.ctypes.data_as(_vp), order.ctypes.data_as(_u32p))
+        xf = np.zeros((ni, 24), np.float32)
+        ids = np.zeros(ni, np.int32)
+        meshes = np.zeros(ni, np.int32)
+        d.orc_instance_export(self.h, xf.ctypes.data_as(_fp), ids.ctypes.data_as(_ip), meshes.ctypes.data_as(_ip))
+        return nodes, order, xf, ids, meshes

@@ -21,6 +21,7 @@
 #include "render/swapchain.h"
 #include "render/wavefront.h"
 #include "scene/scene.h"
+#include "scene/scene_file.h"
 
 #include "oracle_api.h"
 
@@ -287,6 +288,90 @@ void orc_bsdf_eval(const float* p, int n, const float* wo, const float* wi, floa
     out[4 * i] = f.x; out[4 * i + 1] = f.y; out[4 * i + 2] = f.z;
     out[4 * i + 3] = xn::bsdf_pdf_for_nee(a, b, m);
   }
+}
+
+// ---- scene files (reference only; not part of oracle_api.h) --------------
+// load_scene (scene/scene_file.cpp:49-240) into a new handle; cfg5 receives
+// SceneConfig's width, height, samples, min_bounces, max_bounces. NULL when
+// the file does not open.
+void* orc_ref_load_scene_file(const char* path, int* cfg5) {
+  auto* r = new RefScene();
+  xn::SceneConfig cfg;
+  if (!xn::load_scene(path, r->scene, r->camera, cfg)) {
+    delete r;
+    return nullptr;
+  }
+  cfg5[0] = cfg.width; cfg5[1] = cfg.height; cfg5[2] = cfg.samples;
+  cfg5[3] = cfg.min_bounces; cfg5[4] = cfg.max_bounces;
+  r->built = true;
+  return r;
+}
+
+// meshes, materials, lights, directional lights, analytic primitives
+void orc_ref_scene_sizes(void* h, int* out5) {
+  const xn::Scene& s = R(h)->scene;
+  out5[0] = static_cast<int>(s.meshes.size());
+  out5[1] = static_cast<int>(s.materials.size());
+  out5[2] = static_cast<int>(s.lights.size());
+  out5[3] = static_cast<int>(s.dir_lights.size());
+  out5[4] = s.all_prims.size();
+}
+
+void orc_ref_mesh_size(void* h, int mesh, int* nv_nt) {
+  const xn::TriangleMesh& m = R(h)->scene.meshes[mesh];
+  nv_nt[0] = m.num_vertices();
+  nv_nt[1] = m.num_triangles();
+}
+
+void orc_ref_mesh_export(void* h, int mesh, float* pos, float* nrm, int* idx, int* mat) {
+  const xn::TriangleMesh& m = R(h)->scene.meshes[mesh];
+  for (int i = 0; i < m.num_vertices(); ++i) {
+    pos[3 * i] = m.vx[i]; pos[3 * i + 1] = m.vy[i]; pos[3 * i + 2] = m.vz[i];
+    nrm[3 * i] = m.nx[i]; nrm[3 * i + 1] = m.ny[i]; nrm[3 * i + 2] = m.nz[i];
+  }
+  for (int t = 0; t < m.num_triangles(); ++t) {
+    for (int k = 0; k < 3; ++k) idx[3 * t + k] = m.indices[3 * t + k];
+    mat[t] = m.mat_ids[t];
+  }
+}
+
+// authoring parameters in the 16-float order of material_from
+void orc_ref_material_params(void* h, int i, float* p) {
+  const xn::Material& m = R(h)->scene.materials[i];
+  const float v[16] = {m.baseColor.x, m.baseColor.y, m.baseColor.z, m.roughness, m.metallic, m.ior,
+                       m.transmission, m.subsurface, m.subsurfaceColor.x, m.subsurfaceColor.y,
+                       m.subsurfaceColor.z, m.anisotropy, m.emission.x, m.emission.y, m.emission.z, 0.f};
+  std::memcpy(p, v, sizeof v);
+}
+
+// per light: mesh, triangle (as float), emission rgb, area
+void orc_ref_lights_export(void* h, float* out6) {
+  const auto& L = R(h)->scene.lights;
+  for (size_t i = 0; i < L.size(); ++i) {
+    float* o = out6 + 6 * i;
+    o[0] = static_cast<float>(L[i].mesh_id); o[1] = static_cast<float>(L[i].tri_idx);
+    o[2] = L[i].emission.x; o[3] = L[i].emission.y; o[4] = L[i].emission.z; o[5] = L[i].area;
+  }
+}
+
+// per directional light: direction, emission() = color * intensity
+void orc_ref_suns_export(void* h, float* out6) {
+  const auto& D = R(h)->scene.dir_lights;
+  for (size_t i = 0; i < D.size(); ++i) {
+    const xn::Vec3 e = D[i].emission();
+    float* o = out6 + 6 * i;
+    o[0] = D[i].direction.x; o[1] = D[i].direction.y; o[2] = D[i].direction.z;
+    o[3] = e.x; o[4] = e.y; o[5] = e.z;
+  }
+}
+
+// zenith, horizon, ground, sharpness, intensity
+void orc_ref_sky_export(void* h, float* out11) {
+  const xn::Environment& s = R(h)->scene.sky;
+  const float v[11] = {s.zenith_color.x, s.zenith_color.y, s.zenith_color.z, s.horizon_color.x,
+                       s.horizon_color.y, s.horizon_color.z, s.ground_color.x, s.ground_color.y,
+                       s.ground_color.z, s.horizon_sharpness, s.intensity};
+  std::memcpy(out11, v, sizeof v);
 }
 
 }  // extern "C"

@@ -110,3 +110,23 @@ def test_cornell_4096spp(oracle):
     print(f"  4096spp relMSE {e:.3e}; channel mean diff {diff} vs 3 sigma {3 * sigma}", flush=True)
     assert e <= RELMSE_MAX
     assert np.all(diff <= 3 * sigma + 1e-6)
+
+
+def test_scene_file_renders_like_its_description(tmp_path):
+    """A scene written as .xenon + .mat + OBJ and loaded by
+    xnh_scene_load_file renders the same image, bit for bit, as the
+    description it was written from (the loader's parity with the
+    reference's parser is checked on the CPU)."""
+    import scene_files as SF
+    from xenon_amd import HipRenderer, HipScene, scenes
+    desc = scenes.cornell(40, 30, spp=8)
+    desc.suns.append(scenes.Sun((0.3, 1.0, 0.2), (1.0, 0.9, 0.8), 1.5))
+    path = SF.write_scene(desc, str(tmp_path))
+    print(f"[scene file] {path}", flush=True)
+    imgs = []
+    for hs in (HipScene(desc), HipScene.from_file(path)):
+        with HipRenderer(desc.width, desc.height, desc.max_bounces, desc.min_bounces) as r:
+            r.upload_scene(hs)
+            imgs.append(r.render(hs.camera(desc.width, desc.height), 1, desc.spp).copy())
+    assert np.isfinite(imgs[0]).all() and imgs[0].mean() > 0
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))

@@ -27,7 +27,7 @@ CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-D_
              f"-I{ROCM}/include"]
 
 DEVICE_SOURCES = ["device/kernels.hip", "device/guide.hip"]
-HOST_SOURCES = ["host/scene.cpp", "host/scene_api.cpp", "host/api.cpp"]
+HOST_SOURCES = ["host/scene.cpp", "host/scene_api.cpp", "host/scene_io.cpp", "host/api.cpp"]
 
 
 def _run(cmd, verbose):

@@ -123,6 +123,13 @@ class TuneResult(C.Structure):
                 ("ms_per_spp", C.c_float), ("candidates", C.c_int32)]
 
 
+class FileInfo(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("samples", C.c_int32),
+                ("min_bounces", C.c_int32), ("max_bounces", C.c_int32), ("meshes", C.c_int32),
+                ("skipped_primitives", C.c_int32), ("missing_files", C.c_int32),
+                ("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float)]
+
+
 _vp = C.c_void_p
 SIGNATURES = {
     "xnh_create": (C.c_int, [C.POINTER(Config), C.POINTER(_vp)]),
@@ -151,6 +158,8 @@ SIGNATURES = {
     "xnh_scene_build": (C.c_int, [_vp, C.c_int32]),
     "xnh_scene_flat": (C.c_int, [_vp, C.POINTER(FlatScene)]),
     "xnh_camera_look_at": (C.c_int, [_fp, _fp, _fp, C.c_float, C.c_float, C.POINTER(Camera)]),
+    "xnh_scene_load_file": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.POINTER(Camera), C.POINTER(FileInfo)]),
+    "xnh_material_load": (C.c_int, [C.c_char_p, _fp, C.c_char_p, C.c_int32]),
     "xnh_set_tuning": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
     "xnh_get_tuning": (C.c_int, [_vp, _ip, _ip, _ip]),
     "xnh_autotune": (C.c_int, [_vp, C.POINTER(Camera), C.c_int32, C.POINTER(TuneResult)]),

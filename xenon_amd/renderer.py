@@ -67,6 +67,33 @@ class HipScene:
         self._flat = N.FlatScene()
         N.check(L.xnh_scene_flat(self.h, C.byref(self._flat)))
 
+    @classmethod
+    def from_file(cls, path: str) -> "HipScene":
+        """A .xenon scene file (load_scene, scene/scene_file.cpp:49-240) with
+        its .mat materials and OBJ meshes. ``info`` holds the file's render
+        settings and what was skipped (analytic primitives, files that did not
+        open); ``camera()`` uses the file's camera."""
+        self = cls.__new__(cls)
+        self.desc = None
+        h = C.c_void_p()
+        cam = N.Camera()
+        info = N.FileInfo()
+        N.check(N.lib().xnh_scene_load_file(str(path).encode(), C.byref(h), C.byref(cam), C.byref(info)))
+        self.h = h.value
+        self.info = info
+        self.file_camera = cam
+        self._flat = N.FlatScene()
+        N.check(N.lib().xnh_scene_flat(self.h, C.byref(self._flat)))
+        return self
+
+    @property
+    def width(self) -> int:
+        return self.desc.width if self.desc is not None else self.info.width
+
+    @property
+    def height(self) -> int:
+        return self.desc.height if self.desc is not None else self.info.height
+
     def __del__(self):
         try:
             if self.h:
@@ -80,13 +107,18 @@ class HipScene:
         return self._flat
 
     def camera(self, width=None, height=None) -> N.Camera:
-        cam = self.desc.camera
-        W = width or self.desc.width
-        H = height or self.desc.height
+        W = width or self.width
+        H = height or self.height
         aspect = float(np.float32(W) / np.float32(H))
-        e, t, u = (np.array(v, np.float32) for v in (cam.eye, cam.target, cam.up))
+        if self.desc is not None:
+            cam = self.desc.camera
+            e, t, u = (np.array(v, np.float32) for v in (cam.eye, cam.target, cam.up))
+            fov = cam.fov
+        else:
+            e, t, u = (np.array(v[:], np.float32) for v in (self.info.eye, self.info.target, self.info.up))
+            fov = self.info.fov
         out = N.Camera()
-        N.check(N.lib().xnh_camera_look_at(_fp(e), _fp(t), _fp(u), cam.fov, aspect, C.byref(out)))
+        N.check(N.lib().xnh_camera_look_at(_fp(e), _fp(t), _fp(u), fov, aspect, C.byref(out)))
         return out
 
     def blas(self, mesh: int):
