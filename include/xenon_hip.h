@@ -345,13 +345,21 @@ int xnh_camera_look_at(const float* eye3, const float* target3, const float* up3
  * Analytic primitives (sphere, box, disk, quad, plane) are not traced by
  * this backend: they are counted in skipped_primitives, not loaded. Files
  * that cannot be opened are counted in missing_files and skipped (a .mat
- * falls back to the default material), as the reference skips them. */
+ * falls back to the default material), as the reference skips them.
+ * One directive is this backend's own (the reference's parser skips it):
+ *   instance <mesh> <tx> <ty> <tz> <scale> <rx> <ry> <rz> [id]
+ * places mesh number <mesh> (file order) with the TRS of
+ * xnh_scene_add_instance; id defaults to the instance's index. With any
+ * instance line the TLAS holds exactly those instances; without, one
+ * identity instance per mesh, as load_scene builds. A bad instance line
+ * (unknown mesh, missing fields) fails the load with XNH_E_INVALID. */
 typedef struct {
   int32_t width, height, samples;       /* SceneConfig (scene/scene_file.h:11-19) */
   int32_t min_bounces, max_bounces;
   int32_t meshes;                       /* meshes loaded */
   int32_t skipped_primitives;           /* analytic primitives not loaded */
   int32_t missing_files;                /* .mat / .obj files that did not open */
+  int32_t instances;                    /* instance lines (0: one identity instance per mesh) */
   float eye[3], target[3], up[3], fov;  /* the camera directive, for other aspect ratios */
 } xnh_file_info;
 int xnh_scene_load_file(const char* path, xnh_scene** out, xnh_camera* camera, xnh_file_info* info);

@@ -366,3 +366,15 @@ class RefFileScene:
         meshes = np.zeros(ni, np.int32)
         d.orc_instance_export(self.h, xf.ctypes.data_as(_fp), ids.ctypes.data_as(_ip), meshes.ctypes.data_as(_ip))
         return nodes, order, xf, ids, meshes
+
+    def place_instances(self, instances):
+        """Replaces the TLAS by one over (mesh, t3, scale, rot3, id) placements,
+        as orc_scene_build does for explicit instances."""
+        d = self.lib.dll
+        for mesh, t, s, rot, iid in instances:
+            tt = np.array(t, np.float32)
+            rr = np.array(rot, np.float32)
+            d.orc_scene_add_instance(self.h, int(mesh), tt.ctypes.data_as(_fp), float(np.float32(s)),
+                                     rr.ctypes.data_as(_fp), int(iid))
+        if d.orc_scene_build(self.h) != 0:
+            raise RuntimeError("orc_scene_build failed")

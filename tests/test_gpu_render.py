@@ -117,7 +117,7 @@ def test_scene_file_renders_like_its_description(tmp_path):
     xnh_scene_load_file renders the same image, bit for bit, as the
     description it was written from (the loader's parity with the
     reference's parser is checked on the CPU)."""
-    import scene_files as SF
+    from xenon_amd import sceneio as SF
     from xenon_amd import HipRenderer, HipScene, scenes
     desc = scenes.cornell(40, 30, spp=8)
     desc.suns.append(scenes.Sun((0.3, 1.0, 0.2), (1.0, 0.9, 0.8), 1.5))

@@ -6,8 +6,8 @@ triangles and material ids, BLAS and TLAS, area and directional lights and
 the sky. The reference side is oracle/_ref (the reference's own parser,
 compiled from its sources); those tests skip where it is not built. Its
 shipped scene files are read where they lie; the OBJ meshes they name are
-not shipped, so the mesh-bearing scenes are rebuilt with meshes written by
-tests/scene_files.py at the paths the files name."""
+not shipped, so the mesh-bearing scenes run with the stand-ins that
+xenon_amd.sceneio writes at the paths the files name."""
 import ctypes as C
 import os
 import shutil
@@ -15,7 +15,7 @@ import shutil
 import numpy as np
 import pytest
 
-import scene_files as SF
+from xenon_amd import sceneio as SF
 from xenon_amd import HipScene, scenes
 from xenon_amd import native as N
 
@@ -110,22 +110,14 @@ def _ref_scene_files():
 
 def _mirror(tmp_path, rel):
     """The reference scene directory copied under tmp_path/scenes/ (scene
-    file and materials), with every OBJ it names written there from the
-    Cornell box description, so that mesh paths relative to tmp_path (the
-    working directory, as the reference's run script sets it) resolve."""
+    file and materials) and the OBJ files the sample scenes name written
+    under tmp_path by sceneio.write_reference_meshes, so that mesh paths
+    relative to tmp_path (the working directory, as the reference's run
+    script sets it) resolve."""
     d = rel.split("/")[0]
     dst = tmp_path / "scenes" / d
     shutil.copytree(os.path.join(REF_SCENES, d), dst, ignore=shutil.ignore_patterns("*.py"))
-    text = open(os.path.join(REF_SCENES, rel)).read()
-    desc = scenes.cornell()
-    m = desc.meshes[0]
-    names = ["white", "left", "right", "light"]
-    for line in text.splitlines():
-        tok = line.split()
-        if tok and tok[0] == "mesh":
-            obj = tmp_path / tok[1]
-            obj.parent.mkdir(parents=True, exist_ok=True)
-            SF.write_obj(str(obj), m.positions, m.indices, m.mat_ids, names, quads=True)
+    SF.write_reference_meshes(str(tmp_path))
     return str(dst / rel.split("/")[1])
 
 
@@ -136,6 +128,13 @@ def test_reference_scene_files(reference, tmp_path, monkeypatch, rel):
     hs, rs = load_both(reference, path)
     assert hs.info.missing_files == 0
     assert_same_scene(hs, rs)
+    # area lights sit on the stand-in box's light quad
+    f = hs.flat
+    for i in range(f.num_lights):
+        lt = f.lights[i]
+        mv = f.meshes[lt.mesh_id]
+        assert lt.tri_idx < mv.num_triangles
+        assert f.materials[mv.mat_ids[lt.tri_idx]].flags & N_EMISSIVE or "cow" in rel or "lucy" in rel
 
 
 # ---------------------------------------------------------------------------
@@ -288,3 +287,36 @@ def test_material_load(tmp_path):
                                    None, 0)
     assert rc == -1
     assert _bits(p).tolist() == _bits([0.5, 0.5, 0.5, 0.5, 0, 1.5, 0, 0, 1, 1, 1, 0, 0, 0, 0, 0]).tolist()
+
+
+def test_instance_directive(reference, tmp_path, monkeypatch):
+    """`instance` lines (this backend's extension; the reference's parser
+    skips them) build the TLAS the reference builds over the same TRS
+    placements."""
+    desc = scenes.big_mesh(width=8, height=8, level=3)
+    m = desc.meshes[0]
+    SF.write_obj(str(tmp_path / "blob.obj"), m.positions, m.indices)
+    place = [(0, (0, 0, 0), 1.0, (0, 0, 0), 0), (1, (3.5, -1, 2), 0.5, (10, 20, 30), 7),
+             (0, (-4, 2, 1.25), 2.0, (0, 90, 0), 2), (1, (1, 1, 1), 1.5, (-45, 0, 12), 3)]
+    lines = ["mesh blob.obj", "mesh blob.obj 0 1 0 0.75 0 30 0"]
+    for mi, t, s, r, iid in place:
+        tail = f" {iid}" if iid != 2 else ""        # the default id is the line's index
+        lines.append(f"instance {mi} {SF.vec(t)} {SF.g(s)} {SF.vec(r)}{tail}")
+    (tmp_path / "inst.xenon").write_text("\n".join(lines) + "\nlight 1 4 2 2 2 0.5\n")
+    monkeypatch.chdir(tmp_path)
+    hs, rs = load_both(reference, str(tmp_path / "inst.xenon"))
+    assert hs.info.instances == 4 and hs.info.meshes == 2
+    rs.place_instances(place)
+    assert_same_scene(hs, rs)
+    for a, b in zip(hs.tlas(), rs.tlas()):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+
+
+@pytest.mark.parametrize("line", ["instance 2 0 0 0 1 0 0 0", "instance 0 0 0 0 1 0 0", "instance -1 0 0 0 1 0 0 0"])
+def test_instance_directive_errors(tmp_path, line):
+    desc = scenes.big_mesh(width=8, height=8, level=2)
+    m = desc.meshes[0]
+    SF.write_obj(str(tmp_path / "blob.obj"), m.positions, m.indices)
+    (tmp_path / "bad.xenon").write_text(f"mesh {tmp_path / 'blob.obj'}\nmesh {tmp_path / 'blob.obj'}\n{line}\n")
+    with pytest.raises(N.XnhError):
+        HipScene.from_file(str(tmp_path / "bad.xenon"))

@@ -268,7 +268,7 @@ int xnh_scene_load_file(const char* path, xnh_scene** out, xnh_camera* camera, x
   xnh_sky sky = {{0.10f, 0.30f, 0.80f}, {0.55f, 0.70f, 0.85f}, {0.10f, 0.08f, 0.05f}, 3.0f, 1.0f};
   float eye[3] = {0.f, 5.f, 20.f}, target[3] = {0.f, 5.f, 0.f}, up[3] = {0.f, 1.f, 0.f}, fov = 40.f;
   std::map<std::string, int> mats;
-  int nmat = 0, nmesh = 0;
+  int nmat = 0, nmesh = 0, ninst = 0;
   std::string line;
   int rc = XNH_OK;
   auto add_material = [&](const MatParams& m) {
@@ -342,6 +342,22 @@ int xnh_scene_load_file(const char* path, xnh_scene** out, xnh_camera* camera, x
         sky.sharpness = sharp;
       }
       rc = xnh_scene_set_sky(s, sky.zenith, sky.horizon, sky.ground, sky.intensity, sky.sharpness);
+    } else if (cmd == "instance") {
+      // instance <mesh> <tx> <ty> <tz> <scale> <rx> <ry> <rz> [id]: a TRS
+      // placement of a loaded mesh (no counterpart in the reference's
+      // format, which bakes transforms into vertices; its parser skips the
+      // line). With any instance line the TLAS holds exactly these.
+      int32_t mesh = -1;
+      float t[3] = {0.f, 0.f, 0.f}, rot[3] = {0.f, 0.f, 0.f}, scale = 1.f;
+      ss >> mesh >> t[0] >> t[1] >> t[2] >> scale >> rot[0] >> rot[1] >> rot[2];
+      if (!ss || mesh < 0 || mesh >= nmesh) {
+        rc = XNH_E_INVALID;
+        break;
+      }
+      int32_t id = ninst;
+      if (!(ss >> id)) id = ninst;
+      rc = xnh_scene_add_instance(s, mesh, t, scale, rot, id);
+      ++ninst;
     } else if (cmd == "sphere" || cmd == "box" || cmd == "disk" || cmd == "quad" || cmd == "plane") {
       ++fi.skipped_primitives;
     }
@@ -352,6 +368,7 @@ int xnh_scene_load_file(const char* path, xnh_scene** out, xnh_camera* camera, x
     return rc;
   }
   fi.meshes = nmesh;
+  fi.instances = ninst;
   std::memcpy(fi.eye, eye, sizeof eye);
   std::memcpy(fi.target, target, sizeof target);
   std::memcpy(fi.up, up, sizeof up);

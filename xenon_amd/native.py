@@ -126,7 +126,7 @@ class TuneResult(C.Structure):
 class FileInfo(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("samples", C.c_int32),
                 ("min_bounces", C.c_int32), ("max_bounces", C.c_int32), ("meshes", C.c_int32),
-                ("skipped_primitives", C.c_int32), ("missing_files", C.c_int32),
+                ("skipped_primitives", C.c_int32), ("missing_files", C.c_int32), ("instances", C.c_int32),
                 ("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float)]
 
 
