@@ -123,6 +123,8 @@ def test_scene_file_renders_like_its_description(tmp_path):
     desc.suns.append(scenes.Sun((0.3, 1.0, 0.2), (1.0, 0.9, 0.8), 1.5))
     path = SF.write_scene(desc, str(tmp_path))
     print(f"[scene file] {path}", flush=True)
+    info = HipScene.from_file(path).info
+    assert info.missing_files == 0 and info.meshes == 1
     imgs = []
     for hs in (HipScene(desc), HipScene.from_file(path)):
         with HipRenderer(desc.width, desc.height, desc.max_bounces, desc.min_bounces) as r:

@@ -320,3 +320,23 @@ def test_instance_directive_errors(tmp_path, line):
     (tmp_path / "bad.xenon").write_text(f"mesh {tmp_path / 'blob.obj'}\nmesh {tmp_path / 'blob.obj'}\n{line}\n")
     with pytest.raises(N.XnhError):
         HipScene.from_file(str(tmp_path / "bad.xenon"))
+
+
+def test_written_instancing_scene_round_trip(tmp_path):
+    """A description with explicit instances written out (instance lines)
+    loads into the scene the builder makes from the description."""
+    desc = scenes.instancing(width=32, height=18, n_inst=60, tri_range=(300, 1500), n_blas=4, n_emit=4)
+    path = SF.write_scene(desc, str(tmp_path), mesh_root=str(tmp_path))
+    cwd = os.getcwd()
+    try:
+        os.chdir(tmp_path)
+        hs = HipScene.from_file(path)
+    finally:
+        os.chdir(cwd)
+    built = HipScene(desc)
+    assert hs.info.instances == len(desc.instances) and hs.info.missing_files == 0
+    for a, b in zip(built.tlas(), hs.tlas()):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+    for m in range(len(desc.meshes)):
+        for a, b in zip(built.blas(m), hs.blas(m)):
+            assert a.tobytes() == b.tobytes()

@@ -67,11 +67,13 @@ def write_obj(path: str, positions, indices, mat_ids=None, mat_names=None, norma
         f.write("\n".join(out) + "\n")
 
 
-def write_scene(desc, directory: str, name: str = "scene", with_normals: bool = False) -> str:
+def write_scene(desc, directory: str, name: str = "scene", with_normals: bool = False, mesh_root=None) -> str:
     """The description as <directory>/<name>.xenon plus materials/*.mat and
-    meshes/*.obj; mesh paths are written relative to the scene file's
-    directory. Instances other than one identity instance per mesh are not
-    expressible in the format (load_scene bakes transforms into vertices)."""
+    meshes/*.obj. Mesh paths are opened as written, relative to the working
+    directory of the process that loads the file (load_scene does not
+    resolve them against the scene file): they are written absolute, or
+    relative to ``mesh_root`` when given. Explicit instances are written as
+    `instance` lines (this backend's extension of the format)."""
     os.makedirs(os.path.join(directory, "materials"), exist_ok=True)
     os.makedirs(os.path.join(directory, "meshes"), exist_ok=True)
     lines = ["# scene written by xenon_amd.sceneio", f"config {desc.width} {desc.height} {desc.spp}",
@@ -83,10 +85,13 @@ def write_scene(desc, directory: str, name: str = "scene", with_normals: bool = 
         write_mat(os.path.join(directory, "materials", f"{nm}.mat"), nm, m.params16())
         lines.append(f"matfile {nm}.mat")
     for i, m in enumerate(desc.meshes):
-        rel = os.path.join("meshes", f"mesh{i}.obj")
+        obj = os.path.abspath(os.path.join(directory, "meshes", f"mesh{i}.obj"))
         nrm = m.normals if with_normals else None
-        write_obj(os.path.join(directory, rel), m.positions, m.indices, m.mat_ids, names, normals=nrm)
-        lines.append(f"mesh {rel}")
+        write_obj(obj, m.positions, m.indices, m.mat_ids, names, normals=nrm)
+        lines.append(f"mesh {obj if mesh_root is None else os.path.relpath(obj, mesh_root)}")
+    for ins in desc.instances:
+        lines.append(f"instance {ins.mesh} {vec(ins.translation)} {g(ins.scale)} {vec(ins.rot_deg)} "
+                     f"{ins.instance_id}")
     for lt in desc.lights:
         lines.append(f"light {lt.mesh} {lt.tri} {vec(lt.emission)} {g(lt.area)}")
     for s in desc.suns:
