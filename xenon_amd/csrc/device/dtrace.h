@@ -148,14 +148,10 @@ __device__ __forceinline__ InstRec load_inst(const float4* inst, int i) {
 // culls, and box and triangle rounding differ enough for such a subtree to
 // hold a hit. Every box and triangle test therefore sees the reference's
 // t_best, and results and work counts equal its recursive walk, ties
-// included. The visiting order follows the reference's closest-hit walk:
-// BLAS children nearer entry first, ties right first; TLAS test-on-pop with
-// centre-ordered children (tlas.cpp:231-241); a TLAS leaf's instances
-// (pushed with kInstTag, in reverse) walked first to last. Any-hit walks
-// use the same order rather than the reference's left-first one
-// (blas.cpp:303-304, tlas.cpp:285-286): whether a segment is occluded does
-// not depend on the order (every box is tested against the fixed tmax), and
-// nearer-first meets occluders sooner; only the work counts differ. Closest hit starts from t_best = +inf and
+// included. The visiting order follows the reference: BLAS children nearer
+// entry first, ties right first (closest) / left first (any); TLAS
+// test-on-pop with centre-ordered children (tlas.cpp:231-241); a TLAS leaf's
+// instances (pushed with kInstTag, in reverse) walked first to last. Closest hit starts from t_best = +inf and
 // ignores tmax, as Scene::intersect does with a default HitRecord; any hit
 // uses [tmin, tmax]. A stack overflow ends the ray with what it has found.
 constexpr uint32_t kInstTag = 0x80000000u;
@@ -255,7 +251,10 @@ struct Walker {
         const float oa = comp(o, ax), ia = comp(inv, ax);
         const float tl = (nc.x - oa) * ia;
         const float tr = (nc.y - oa) * ia;
-        if (tl < tr) {   // centre order (also for any hit, see interior_step)
+        if (kAny) {
+          st.push(child);
+          st.push(child + 1);
+        } else if (tl < tr) {
           st.push(child + 1);
           st.push(child);
         } else {
@@ -285,10 +284,8 @@ struct Walker {
     const uint32_t er = blas_entry(child + 1, __float_as_uint(rb.z), __float_as_uint(rb.w));
     uint32_t next;
     if (m == 3) {
-      // nearer child first, ties right first (closest-hit order of
-      // blas.cpp:252-258; also used for any hit, whose answer does not
-      // depend on the order and whose occluders are found sooner)
-      const bool left_first = tn0 < tn1;
+      // closest: nearer first, ties right first; any: left first
+      const bool left_first = kAny || tn0 < tn1;
       st.push(left_first ? er : el);
       next = left_first ? el : er;
     } else if (m) {
