@@ -89,8 +89,14 @@ def test_any_bit_exact(pair):
     print(f"  {desc.name}: {len(srays)} shadow rays, occluded {int(occ_ref.sum())}, mismatches {bad}", flush=True)
     assert bad / len(srays) <= GRAZING_MAX
     if bad == 0:
-        _check_work([st["shadow_node_tests"], st["shadow_interior"], st["shadow_tri_tests"],
-                     st["shadow_inst_visits"]], cnt)
+        # the device walks any-hit BLAS children farther first (the answer
+        # does not depend on the order): work differs from the oracle's
+        # left-first walk, but not by much
+        dev = [st["shadow_node_tests"], st["shadow_interior"], st["shadow_tri_tests"], st["shadow_inst_visits"]]
+        ref = [int(x) for x in cnt]
+        print(f"  any-hit work node/interior/tri/inst: device {dev}, oracle (left first) {ref}", flush=True)
+        for k in range(4):
+            assert dev[k] <= 2 * ref[k] + 64, k
 
 
 def test_empty_batch_and_errors(pair):

@@ -148,10 +148,13 @@ __device__ __forceinline__ InstRec load_inst(const float4* inst, int i) {
 // culls, and box and triangle rounding differ enough for such a subtree to
 // hold a hit. Every box and triangle test therefore sees the reference's
 // t_best, and results and work counts equal its recursive walk, ties
-// included. The visiting order follows the reference: BLAS children nearer
-// entry first, ties right first (closest) / left first (any); TLAS
-// test-on-pop with centre-ordered children (tlas.cpp:231-241); a TLAS leaf's
-// instances (pushed with kInstTag, in reverse) walked first to last. Closest hit starts from t_best = +inf and
+// included. Closest-hit rays follow the reference's visiting order: BLAS
+// children nearer entry first, ties right first; TLAS test-on-pop with
+// centre-ordered children (tlas.cpp:231-241); a TLAS leaf's instances
+// (pushed with kInstTag, in reverse) walked first to last. Any-hit rays
+// take BLAS children farther first (the reference: left first,
+// blas.cpp:303-304); the occlusion answer does not depend on the order, only
+// the work does. Closest hit starts from t_best = +inf and
 // ignores tmax, as Scene::intersect does with a default HitRecord; any hit
 // uses [tmin, tmax]. A stack overflow ends the ray with what it has found.
 constexpr uint32_t kInstTag = 0x80000000u;
@@ -284,8 +287,12 @@ struct Walker {
     const uint32_t er = blas_entry(child + 1, __float_as_uint(rb.z), __float_as_uint(rb.w));
     uint32_t next;
     if (m == 3) {
-      // closest: nearer first, ties right first; any: left first
-      const bool left_first = kAny || tn0 < tn1;
+      // closest: nearer first, ties right first (blas.cpp:252-258). Any
+      // hit: farther first. Occlusion does not depend on the order (every
+      // box is tested against the fixed tmax); a shadow ray's nearer child
+      // is mostly the box around its own origin, whose triangles it cannot
+      // hit, so the farther child meets occluders sooner.
+      const bool left_first = kAny ? !(tn0 < tn1) : tn0 < tn1;
       st.push(left_first ? er : el);
       next = left_first ? el : er;
     } else if (m) {
