@@ -15,6 +15,22 @@ struct f3 {
   float x, y, z;
 };
 
+// Loads through pointers known to address device (global) memory. Pointers
+// read from memory (scene tables in the job list) are otherwise generic, and
+// generic loads count against the LDS wait counter as well, so every LDS
+// stack access would also wait for outstanding node fetches.
+typedef float xv4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gld4(const float4* p, size_t i) {
+  const xv4 v = ((const __attribute__((address_space(1))) xv4*)p)[i];
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint32_t gld1(const uint32_t* p, size_t i) {
+  return ((const __attribute__((address_space(1))) uint32_t*)p)[i];
+}
+__device__ __forceinline__ void gst1(uint32_t* p, size_t i, uint32_t v) {
+  ((__attribute__((address_space(1))) uint32_t*)p)[i] = v;
+}
+
 __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 splat(float s) { return f3{s, s, s}; }
 __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }

@@ -34,21 +34,35 @@ extern __shared__ uint32_t trav_lds[];
 
 struct TravStack {
   int lds_col;          // this lane's column: trav_lds[e * lds_stride + lds_col]
+  int lds_off;          // sp * lds_stride + lds_col (kept as a running index)
   uint32_t* spill;      // this lane's spill column: spill[(e - lds_depth) * spill_stride]
-  int lds_stride, spill_stride, lds_depth, sp;
+  int lds_stride, spill_stride, lds_depth, sp;   // lds_depth >= 1
   bool overflow;
+  __device__ __forceinline__ void reset() {
+    sp = 0;
+    lds_off = lds_col;
+    overflow = false;
+  }
   __device__ __forceinline__ void push(uint32_t v) {
     if (sp >= kMaxStack) {
       overflow = true;
       return;
     }
-    if (sp < lds_depth) trav_lds[sp * lds_stride + lds_col] = v;
-    else spill[(sp - lds_depth) * spill_stride] = v;
+    if (sp < lds_depth) trav_lds[lds_off] = v;
+    else gst1(spill, static_cast<size_t>(sp - lds_depth) * spill_stride, v);
     ++sp;
+    lds_off += lds_stride;
   }
+  // The LDS read is issued unconditionally (at this lane's entry 0 when the
+  // entry lies in the spill) so that the common case compiles to a plain
+  // LDS access rather than a generic-address load.
   __device__ __forceinline__ uint32_t pop() {
     --sp;
-    return (sp < lds_depth) ? trav_lds[sp * lds_stride + lds_col] : spill[(sp - lds_depth) * spill_stride];
+    lds_off -= lds_stride;
+    const bool in_lds = sp < lds_depth;
+    uint32_t v = trav_lds[in_lds ? lds_off : lds_col];
+    if (!in_lds) v = gld1(spill, static_cast<size_t>(sp - lds_depth) * spill_stride);
+    return v;
   }
 };
 
@@ -68,7 +82,7 @@ struct HitRef {
 // Moller-Trumbore on a pre-gathered triangle (geometry/primitives.h:25-49).
 __device__ __forceinline__ bool tri_test(const float4* __restrict__ tris, uint32_t slot, f3 o, f3 d, float tmin,
                                          float tmax, float& t, float& u, float& v) {
-  const float4 a = tris[3 * slot], b = tris[3 * slot + 1], c = tris[3 * slot + 2];
+  const float4 a = gld4(tris, 3 * slot), b = gld4(tris, 3 * slot + 1), c = gld4(tris, 3 * slot + 2);
   const f3 v0{a.x, a.y, a.z}, e1{a.w, b.x, b.y}, e2{b.z, b.w, c.x};
   const f3 h = cross(d, e2);
   const float det = dot(e1, h);
@@ -102,6 +116,34 @@ __device__ __forceinline__ int pair_test(float4 la, float4 lb, float4 ra, float4
   return (ln <= lf ? 1 : 0) | (rn <= rf ? 2 : 0);
 }
 
+// pair_test with the hardware min/max (v_min3/v_max3 chains instead of
+// compare-and-select pairs). Equal to pair_test whenever no slab distance is
+// NaN, and a NaN needs 0 * inf, so it is used only for rays whose origin,
+// direction and inverse direction are finite in every component (box
+// coordinates are finite). Without NaNs the two differ at
+// most in the sign of a zero distance, which no comparison distinguishes.
+__device__ __forceinline__ int pair_test_fast(float4 la, float4 lb, float4 ra, float4 rb, f3 inv, f3 o, float tmin,
+                                              float tmax, float& tn0, float& tn1) {
+  const float lx0 = (la.x - o.x) * inv.x, lx1 = (la.w - o.x) * inv.x;
+  const float ly0 = (la.y - o.y) * inv.y, ly1 = (lb.x - o.y) * inv.y;
+  const float lz0 = (la.z - o.z) * inv.z, lz1 = (lb.y - o.z) * inv.z;
+  const float rx0 = (ra.x - o.x) * inv.x, rx1 = (ra.w - o.x) * inv.x;
+  const float ry0 = (ra.y - o.y) * inv.y, ry1 = (rb.x - o.y) * inv.y;
+  const float rz0 = (ra.z - o.z) * inv.z, rz1 = (rb.y - o.z) * inv.z;
+  const float ln = fmaxf(fmaxf(fmaxf(fminf(lx0, lx1), fminf(ly0, ly1)), fminf(lz0, lz1)), tmin);
+  const float lf = fminf(fminf(fminf(fmaxf(lx0, lx1), fmaxf(ly0, ly1)), fmaxf(lz0, lz1)), tmax);
+  const float rn = fmaxf(fmaxf(fmaxf(fminf(rx0, rx1), fminf(ry0, ry1)), fminf(rz0, rz1)), tmin);
+  const float rf = fminf(fminf(fminf(fmaxf(rx0, rx1), fmaxf(ry0, ry1)), fmaxf(rz0, rz1)), tmax);
+  tn0 = ln;
+  tn1 = rn;
+  return (ln <= lf ? 1 : 0) | (rn <= rf ? 2 : 0);
+}
+
+__device__ __forceinline__ bool all_finite(f3 v) {
+  return fabsf(v.x) < __builtin_huge_valf() && fabsf(v.y) < __builtin_huge_valf() &&
+         fabsf(v.z) < __builtin_huge_valf();
+}
+
 __device__ __forceinline__ uint32_t blas_entry(uint32_t node, uint32_t child, uint32_t count) {
   if (count == 0) return child;
   if (count < kEntBigLeaf) return (count << kEntShift) | child;
@@ -117,7 +159,7 @@ struct InstRec {
   int32_t lmap;
 };
 __device__ __forceinline__ InstRec load_inst(const float4* inst, int i) {
-  const float4 a = inst[4 * i], b = inst[4 * i + 1], c = inst[4 * i + 2], m = inst[4 * i + 3];
+  const float4 a = gld4(inst, 4 * i), b = gld4(inst, 4 * i + 1), c = gld4(inst, 4 * i + 2), m = gld4(inst, 4 * i + 3);
   InstRec r;
   r.c0 = f3{a.x, a.y, a.z};
   r.c1 = f3{a.w, b.x, b.y};
@@ -173,6 +215,7 @@ struct Walker {
   uint32_t e;         // next BLAS entry to process, or kEntEnd; a leaf only while `leaf` is held
   uint32_t leaf;      // held leaf entry (tested before e), 0 = none
   bool inst_found, found;
+  bool fast;          // lo, ld, linv finite: pair_test_fast is exact
   bool stuck;         // the round bound was reached (the walk was abandoned)
   HitRef hit;         // u, v, slot, inst of the closest hit; its t is tbest
 
@@ -188,10 +231,10 @@ struct Walker {
     leaf = 0u;
     inst_found = false;
     found = false;
+    fast = false;
     stuck = false;
     hit = HitRef{0.f, 0.f, 0.f, 0u, -1};
-    st.sp = 0;
-    st.overflow = false;
+    st.reset();
     if (S.num_tnodes > 0) st.push(0u);
   }
 
@@ -217,6 +260,7 @@ struct Walker {
         lo = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
         ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
         linv = f3{1.f / ld.x, 1.f / ld.y, 1.f / ld.z};
+        fast = all_finite(linv) && all_finite(ld) && all_finite(lo);
         inst = ii;
         inst_found = false;
         blas_base = st.sp;
@@ -231,7 +275,7 @@ struct Walker {
         return false;
       }
       const uint32_t ni = te;
-      const float4 na = S.tnodes[3 * ni], nb = S.tnodes[3 * ni + 1];
+      const float4 na = gld4(S.tnodes, 3 * ni), nb = gld4(S.tnodes, 3 * ni + 1);
       if (kCount) ++c.node_tests;
       // sign-selected slab (geometry/aabb.h:59-71): near/far by direction sign
       const bool sx = d.x < 0.f, sy = d.y < 0.f, sz = d.z < 0.f;
@@ -246,10 +290,10 @@ struct Walker {
       if (!(lo_t <= hi_t)) continue;
       const uint32_t child = __float_as_uint(nb.z), count = __float_as_uint(nb.w);
       if (count > 0) {
-        for (uint32_t k = count; k-- > 0;) st.push(kInstTag | S.torder[child + k]);
+        for (uint32_t k = count; k-- > 0;) st.push(kInstTag | gld1(S.torder, child + k));
       } else {
         if (kCount) ++c.interior;
-        const float4 nc = S.tnodes[3 * ni + 2];
+        const float4 nc = gld4(S.tnodes, 3 * ni + 2);
         const int ax = static_cast<int>(__float_as_uint(nc.z));
         const float oa = comp(o, ax), ia = comp(inv, ax);
         const float tl = (nc.x - oa) * ia;
@@ -273,16 +317,18 @@ struct Walker {
   // One interior step on e (an interior entry, no leaf held): pair-test its
   // children, continue with the nearer hit one, push the farther; a leaf
   // reached is held and the next entry popped (walked after the leaf).
+  template <bool kFast>
   __device__ __forceinline__ void interior_step(const DevScene& S, TravStack& st, TravCount& c) {
     const uint32_t child = e;
-    const float4 la = S.bnodes[2 * child], lb = S.bnodes[2 * child + 1];
-    const float4 ra = S.bnodes[2 * child + 2], rb = S.bnodes[2 * child + 3];
+    const float4 la = gld4(S.bnodes, 2 * child), lb = gld4(S.bnodes, 2 * child + 1);
+    const float4 ra = gld4(S.bnodes, 2 * child + 2), rb = gld4(S.bnodes, 2 * child + 3);
     float tn0, tn1;
     if (kCount) {
       ++c.interior;
       c.node_tests += 2;
     }
-    const int m = pair_test(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1);
+    const int m = kFast ? pair_test_fast(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1)
+                        : pair_test(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1);
     const uint32_t el = blas_entry(child, __float_as_uint(lb.z), __float_as_uint(lb.w));
     const uint32_t er = blas_entry(child + 1, __float_as_uint(rb.z), __float_as_uint(rb.w));
     uint32_t next;
@@ -311,7 +357,7 @@ struct Walker {
   __device__ __forceinline__ bool leaf_step(const DevScene& S, TravStack& st, TravCount& c) {
     uint32_t count = leaf >> kEntShift, first = leaf & kEntIndexMask;
     if (count == kEntBigLeaf) {
-      const float4 nb = S.bnodes[2 * first + 1];
+      const float4 nb = gld4(S.bnodes, 2 * first + 1);
       first = __float_as_uint(nb.z);
       count = __float_as_uint(nb.w);
     }
@@ -341,14 +387,23 @@ struct Walker {
   // One round; true once the ray is finished (closest: `found` / `hit`;
   // any: `found` = occluded). Reached by the lanes that hold a ray; the
   // phases are wave-wide over those lanes.
-  __device__ __forceinline__ bool round(const DevScene& S, TravStack& st, TravCount& c) {
-    if (tlas_steps(S, st, c)) return true;
+  // Interior steps until no lane of the wave has an interior entry to walk.
+  template <bool kFast>
+  __device__ __forceinline__ void interior_phase(const DevScene& S, TravStack& st, TravCount& c) {
     for (;;) {
       const bool walk = leaf == 0u && ent_interior(e);
       if (__ballot(walk) == 0ull) break;
-      if (walk) interior_step(S, st, c);
+      if (walk) interior_step<kFast>(S, st, c);
       if (st.overflow) break;
     }
+  }
+
+  __device__ __forceinline__ bool round(const DevScene& S, TravStack& st, TravCount& c) {
+    if (tlas_steps(S, st, c)) return true;
+    // the hardware min/max slab test when every walking lane's ray allows it
+    // (a wave-uniform choice: one loop body per phase)
+    if (__ballot(leaf == 0u && ent_interior(e) && !fast) == 0ull) interior_phase<true>(S, st, c);
+    else interior_phase<false>(S, st, c);
     if (st.overflow) {
       end_instance();
       return true;

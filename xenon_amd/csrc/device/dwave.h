@@ -61,8 +61,7 @@ __device__ __forceinline__ TravStack make_stack(uint32_t* spill, int lds_depth) 
   st.spill = spill + (static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x);
   st.spill_stride = static_cast<int>(gridDim.x * blockDim.x);
   st.lds_depth = lds_depth;
-  st.sp = 0;
-  st.overflow = false;
+  st.reset();
   return st;
 }
 
