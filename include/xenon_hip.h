@@ -231,8 +231,8 @@ typedef struct {
   float guide_fraction;    /* one-sample MIS: probability of sampling the guide (0: 0.5) */
   uint32_t seed;           /* parameter initialisation */
   float bounds_lo[3], bounds_hi[3];   /* scene box mapped to the unit cube */
-  int32_t train_batch;     /* training samples per step (0: 2^18) */
-  int32_t record_paths;    /* paths per batch that record training samples (0: 2^16) */
+  int32_t train_batch;     /* training samples per batch (0: 2^17) */
+  int32_t record_paths;    /* paths per batch that record training samples (0: all) */
   int32_t train_steps;     /* optimisation steps per batch, each on an equal slice of its records (0: 1) */
   int32_t warmup_steps;    /* the guide's MIS probability ramps from 0 to guide_fraction over this many steps */
   int32_t guide_depth;     /* guide the bounces below this depth (0: 3); deeper vertices sample the BSDF */
@@ -241,8 +241,10 @@ typedef struct {
 /* Turns guided rendering on for this context with fresh parameters. From
  * then on every render samples exact-Lambertian surfaces by one-sample MIS
  * of the guide (probability guide_fraction) and the BSDF, and after every
- * batch trains the network on up to train_batch vertices of about
- * record_paths paths of that batch (all on the device). */
+ * batch trains the network on up to train_batch of the guided vertices of
+ * record_paths paths of that batch, drawn in proportion to their loss
+ * weight (all on the device). At depth 0 a pixel's samples within a batch
+ * share the mixture of its first sample. */
 int xnh_guide_enable(xnh_ctx* ctx, const xnh_guide_config* cfg);
 int xnh_guide_disable(xnh_ctx* ctx);
 /* 1: train after every batch (default); 0: render with the network frozen. */

@@ -144,10 +144,12 @@ __device__ __forceinline__ bool all_finite(f3 v) {
          fabsf(v.z) < __builtin_huge_valf();
 }
 
+// Written with selects rather than branches: it runs for both children of
+// every interior step.
 __device__ __forceinline__ uint32_t blas_entry(uint32_t node, uint32_t child, uint32_t count) {
-  if (count == 0) return child;
-  if (count < kEntBigLeaf) return (count << kEntShift) | child;
-  return (kEntBigLeaf << kEntShift) | node;
+  const uint32_t c = min(count, kEntBigLeaf);
+  const uint32_t leaf = (c << kEntShift) | (count < kEntBigLeaf ? child : node);
+  return count == 0u ? child : leaf;
 }
 
 // Instance: world_to_local columns and metadata (see dev_types.h).

@@ -91,30 +91,52 @@ __device__ __forceinline__ void sh16(f3 n, float* o) {
   o[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
 }
 
-// Writes the 48 FP16 encoding values of (p, n) to `row`.
+// Writes the 48 FP16 encoding values of (p, n) to `row`. The corner
+// fetches of four levels at a time are issued before any is used, so a wave
+// waits on memory four times per point rather than sixteen; each level still
+// sums its corners in order 0..7.
 __device__ __forceinline__ void guide_encode(const GuideNet& net, f3 p, f3 n, _Float16* row) {
   const f3 q = guide_unit_pos(net, p);
-  const uint32_t T = 1u << net.t_log2;
-  for (int l = 0; l < kGLevels; ++l) {
-    float a0 = 0.f, a1 = 0.f;
-    const _Float16* g = reinterpret_cast<const _Float16*>(net.grid) + static_cast<size_t>(l) * T * 2u;
-    for (int c = 0; c < 8; ++c) {
+  const size_t T = static_cast<size_t>(1) << net.t_log2;
+  const uint32_t* g32 = reinterpret_cast<const uint32_t*>(net.grid);   // the two features of a slot
+  for (int l0 = 0; l0 < kGLevels; l0 += 4) {
+    uint32_t pr[32];
+    float w[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) {
+      const int l = l0 + (k >> 3);
       uint32_t s;
-      float w;
-      grid_corner(net, q, l, c, s, w);
-      const uint32_t pair = *reinterpret_cast<const uint32_t*>(g + 2u * s);
-      _Float16 h0, h1;
-      __builtin_memcpy(&h0, &pair, 2);
-      __builtin_memcpy(&h1, reinterpret_cast<const char*>(&pair) + 2, 2);
-      a0 = a0 + w * static_cast<float>(h0);
-      a1 = a1 + w * static_cast<float>(h1);
+      grid_corner(net, q, l, k & 7, s, w[k]);
+      pr[k] = gld1(g32, static_cast<size_t>(l) * T + s);
     }
-    row[2 * l] = static_cast<_Float16>(a0);
-    row[2 * l + 1] = static_cast<_Float16>(a1);
+#pragma unroll
+    for (int lk = 0; lk < 4; ++lk) {
+      float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const uint32_t pair = pr[8 * lk + c];
+        _Float16 h0, h1;
+        __builtin_memcpy(&h0, &pair, 2);
+        __builtin_memcpy(&h1, reinterpret_cast<const char*>(&pair) + 2, 2);
+        a0 = a0 + w[8 * lk + c] * static_cast<float>(h0);
+        a1 = a1 + w[8 * lk + c] * static_cast<float>(h1);
+      }
+      row[2 * (l0 + lk)] = static_cast<_Float16>(a0);
+      row[2 * (l0 + lk) + 1] = static_cast<_Float16>(a1);
+    }
   }
   float s[16];
   sh16(n, s);
   for (int k = 0; k < 16; ++k) row[32 + k] = static_cast<_Float16>(s[k]);
+}
+
+// LDS written by some lanes of a wave and read by others: the wave's LDS
+// operations complete in order, so a compiler fence at wavefront scope is
+// enough (no work-group barrier, which would need every wave present).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // ---- MFMA tiles -----------------------------------------------------------------
@@ -181,33 +203,47 @@ __device__ __forceinline__ void dense_reg(const h4v* x, const h4v* w, h4v* y) {
   }
 }
 
-// wave: forward pass of 64 points, one per lane (p: world position, n:
-// normal). lds: 2 x 64 x kGIn FP16 scratch of this wave. On return the
-// lane's 48 raw outputs (FP32 accumulation, stored FP16) are at
-// lds + 64 kGIn + lane kGIn.
-__device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3 n, _Float16* lds) {
-  const int lane = static_cast<int>(threadIdx.x & 63u), g = lane >> 4, li = lane & 15;
-  guide_encode(net, p, n, lds + lane * kGIn);
+// Copies the tiled FP16 weights (GuideNet::wt, kWTotal values) into LDS;
+// every thread of the block must call it.
+constexpr int kWTiles = kWTotal / 4;   // h4v operands
+__device__ __forceinline__ void stage_weights(const uint16_t* wt, h4v* wl) {
+  typedef uint16_t u4v __attribute__((ext_vector_type(4)));
+  const u4v* src = reinterpret_cast<const u4v*>(wt);
+  for (int k = static_cast<int>(threadIdx.x); k < kWTiles; k += static_cast<int>(blockDim.x)) {
+    const u4v v = src[k];
+    h4v h;
+    __builtin_memcpy(&h, &v, sizeof h);
+    wl[k] = h;
+  }
   __syncthreads();
+}
+
+// wave: forward pass of 64 points, one per lane (p: world position, n:
+// normal). act: this wave's 64 x kGIn FP16 LDS rows; wl: the weights in LDS
+// (stage_weights). On return the lane's 48 raw outputs (FP32 accumulation,
+// stored FP16) are at act + lane kGIn.
+__device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3 n, _Float16* act, const h4v* wl) {
+  const int lane = static_cast<int>(threadIdx.x & 63u), g = lane >> 4, li = lane & 15;
+  guide_encode(net, p, n, act + lane * kGIn);
+  wave_lds_sync();
   h4v x[4][3];
   for (int nt = 0; nt < 4; ++nt)
     for (int kt = 0; kt < 3; ++kt)
-      x[nt][kt] = *reinterpret_cast<const h4v*>(lds + (16 * nt + li) * kGIn + 16 * kt + 4 * g);
-  _Float16* out = lds + 64 * kGIn;
-  const h4v* wt = reinterpret_cast<const h4v*>(net.wt);
+      x[nt][kt] = *reinterpret_cast<const h4v*>(act + (16 * nt + li) * kGIn + 16 * kt + 4 * g);
+  wave_lds_sync();   // every row is read before outputs overwrite them
   for (int nt = 0; nt < 4; ++nt) {
     h4v a[4], b[4];
-    dense_reg<3, 4>(x[nt], wt + kW1 / 4, a);
-    dense_reg<4, 4>(a, wt + kW2 / 4, b);
-    dense_reg<4, 4>(b, wt + kW3 / 4, a);
+    dense_reg<3, 4>(x[nt], wl + kW1 / 4, a);
+    dense_reg<4, 4>(a, wl + kW2 / 4, b);
+    dense_reg<4, 4>(b, wl + kW3 / 4, a);
     for (int mt = 0; mt < kGOut / 16; ++mt) {
       f4v acc = {0.f, 0.f, 0.f, 0.f};
       for (int kt = 0; kt < 4; ++kt)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wt[kW4 / 4 + (mt * 4 + kt) * 64 + lane], a[kt], acc, 0, 0, 0);
-      for (int j = 0; j < 4; ++j) out[(16 * nt + li) * kGIn + 16 * mt + 4 * g + j] = static_cast<_Float16>(acc[j]);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wl[kW4 / 4 + (mt * 4 + kt) * 64 + lane], a[kt], acc, 0, 0, 0);
+      for (int j = 0; j < 4; ++j) act[(16 * nt + li) * kGIn + 16 * mt + 4 * g + j] = static_cast<_Float16>(acc[j]);
     }
   }
-  __syncthreads();
+  wave_lds_sync();
 }
 
 // wave: the full forward pass; lds_x holds the encodings (64 x 48).

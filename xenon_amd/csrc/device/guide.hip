@@ -16,6 +16,8 @@
 // over the batch (FP32 atomics); the caller divides by the batch size in Adam.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dbsdf.h"
 #include "dwave.h"
 #include "guide.h"
@@ -37,6 +39,40 @@ __device__ __forceinline__ unsigned long long g_wave_sum(unsigned long long v) {
 __device__ __forceinline__ float g_wave_sumf(float v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
+}
+
+// Block-wide sum (blockDim.x a multiple of 64, at most 1024); every thread
+// gets it. red: blockDim.x / 64 floats of LDS.
+__device__ __forceinline__ float block_sumf(float v, float* red) {
+  v = g_wave_sumf(v);
+  const int w = static_cast<int>(threadIdx.x >> 6), nw = static_cast<int>(blockDim.x >> 6);
+  if (glane() == 0) red[w] = v;
+  __syncthreads();
+  float t = 0.f;
+  for (int k = 0; k < nw; ++k) t += red[k];
+  __syncthreads();
+  return t;
+}
+
+// Block-wide exclusive prefix sum of x; *total gets the block sum. red:
+// blockDim.x / 64 + 1 words of LDS.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* red, uint32_t* total) {
+  const int lane = glane(), w = static_cast<int>(threadIdx.x >> 6), nw = static_cast<int>(blockDim.x >> 6);
+  uint32_t inc = x;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(inc, off);
+    if (lane >= off) inc += y;
+  }
+  if (lane == 63) red[w] = inc;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int k = 0; k < nw; ++k) {
+    if (k < w) before += red[k];
+    all += red[k];
+  }
+  __syncthreads();
+  *total = all;
+  return before + inc - x;
 }
 
 // Output gradient of one sample; raw: 48 outputs; g: 48 FP32 gradients.
@@ -115,14 +151,16 @@ struct TrainLds {
 // Evaluates the network for n points: raw outputs (n x 48, FP32).
 __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const float* pos, const float* nrm,
                                                    float* raw) {
-  __shared__ _Float16 lds[2 * 64 * kGIn];
+  __shared__ _Float16 lds[64 * kGIn];
+  __shared__ h4v wl[kWTiles];
+  stage_weights(net.wt, wl);
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
   const bool ok = i < n;
   const f3 p = ok ? f3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]} : f3{net.lo[0], net.lo[1], net.lo[2]};
   const f3 nn = ok ? f3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]} : f3{0.f, 0.f, 1.f};
-  guide_forward_wave(net, p, nn, lds);
-  const _Float16* out = lds + 64 * kGIn + lane * kGIn;
+  guide_forward_wave(net, p, nn, lds, wl);
+  const _Float16* out = lds + lane * kGIn;
   if (ok)
     for (int k = 0; k < kGOut; ++k) raw[static_cast<size_t>(i) * kGOut + k] = static_cast<float>(out[k]);
 }
@@ -132,16 +170,18 @@ __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const fl
 // dir; then writes the mixture pdf at dir[i] (drawn or given) to pdf.
 __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const float* pos, const float* nrm,
                                                     const float* u, float* dir, float* pdf) {
-  __shared__ _Float16 lds[2 * 64 * kGIn];
+  __shared__ _Float16 lds[64 * kGIn];
+  __shared__ h4v wl[kWTiles];
+  stage_weights(net.wt, wl);
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
   const bool ok = i < n;
   const f3 p = ok ? f3{pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]} : f3{net.lo[0], net.lo[1], net.lo[2]};
   const f3 nn = ok ? f3{nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]} : f3{0.f, 0.f, 1.f};
-  guide_forward_wave(net, p, nn, lds);
+  guide_forward_wave(net, p, nn, lds, wl);
   if (!ok) return;
   float raw[kGOut];
-  for (int k = 0; k < kGLobes * 5; ++k) raw[k] = static_cast<float>(lds[64 * kGIn + lane * kGIn + k]);
+  for (int k = 0; k < kGLobes * 5; ++k) raw[k] = static_cast<float>(lds[lane * kGIn + k]);
   Mixture m;
   decode_mixture(raw, m);
   f3 w;
@@ -165,23 +205,40 @@ __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const f
 // normal input is the geometric normal) and stores the mixture for k_shade.
 // One wave per block, grid-stride over the batch's 64-entry units. Launched
 // for the bounces that are guided (below the configured guide depth).
-__global__ void __launch_bounds__(64) k_guide_infer(WaveBuffers B, int qi, GuideNet net) {
+// Guide mixtures for the hits of queue qi (after its closest-hit pass): one
+// stored mixture per entry whose hit is a non-emitting exact-Lambertian
+// surface; every other entry gets an invalid mark (a negative first lobe
+// weight), which the shading kernel checks. At depth 0 the queue is in
+// camera-ray order (entry = sample * npix + pixel within a chunk) and only
+// each pixel's first sample of the batch is evaluated: the pixel's other
+// samples use that mixture as their guide too (a proposal that does not
+// depend on their own random numbers, so the estimate stays unbiased).
+constexpr int kInferWaves = 4;   // waves per work-group of k_guide_infer
+
+__global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(WaveBuffers B, int qi, int depth, GuideNet net) {
   __shared__ BatchView v;
-  __shared__ _Float16 lds[2 * 64 * kGIn];
+  __shared__ _Float16 act[kInferWaves][64 * kGIn];
+  __shared__ h4v wl[kWTiles];
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
   load_batch(v, B, counts, false);
+  stage_weights(net.wt, wl);
   const uint32_t total = v.prefix[v.nch];
   const PathQueue q = B.q[qi];
   const uint32_t C = static_cast<uint32_t>(B.capacity);
-  for (uint32_t unit = blockIdx.x; unit < total; unit += gridDim.x) {
+  const uint32_t wid = threadIdx.x >> 6;
+  // waves work independently from here on (no work-group barrier below)
+  for (uint32_t unit = blockIdx.x * kInferWaves + wid; unit < total; unit += gridDim.x * kInferWaves) {
     const int ci = uniform(find_chunk(v, unit));
     const DevChunk& ck = v.ch[ci];
+    const uint32_t first = (unit - v.prefix[ci]) * 64u;
+    const uint32_t lim = depth == 0 ? min(v.cnt[ci], static_cast<uint32_t>(ck.npix)) : v.cnt[ci];
+    if (first >= lim) continue;   // uniform: a later sample of every pixel in the unit
     const DevScene& S = B.jobs[uniform(ck.job)].S;
-    const uint32_t off = (unit - v.prefix[ci]) * 64u + static_cast<uint32_t>(glane());
+    const uint32_t off = first + static_cast<uint32_t>(glane());
     const uint32_t i = static_cast<uint32_t>(ck.base) + off;
     bool want = false;
     f3 p{net.lo[0], net.lo[1], net.lo[2]}, n{0.f, 0.f, 1.f};
-    if (off < v.cnt[ci]) {
+    if (off < lim) {
       const float4 hv = B.hit[i];
       const int inst = __float_as_int(hv.w);
       if (inst >= 0) {
@@ -195,49 +252,110 @@ __global__ void __launch_bounds__(64) k_guide_infer(WaveBuffers B, int qi, Guide
           n = sf.gn;
         }
       }
+      if (!want) B.mix[8 * C + i] = make_float4(-1.f, 0.f, 0.f, 0.f);
     }
     if (__ballot(want) == 0ull) continue;
-    guide_forward_wave(net, p, n, lds);
-    if (want) decode_store_mixture(lds + 64 * kGIn + glane() * kGIn, B.mix, C, i);
+    guide_forward_wave(net, p, n, act[wid], wl);
+    if (want) decode_store_mixture(act[wid] + glane() * kGIn, B.mix, C, i);
   }
 }
 
-// After a guided batch: turns its training records into training arrays
-// (position, normal, direction, lambda), adds the lambdas to *lam_sum and
-// writes the record count to *n_out.
+// Lambda of record r (0 when its path never wrote the prefix).
 // lambda = luminance((L - prefix) / T): the radiance the path gathered after
 // the vertex over the throughput that reached it, i.e. f cos / p times the
 // incident radiance along the sampled direction (the weight of the KL
-// objective of Mueller et al. 2019). Records whose prefix was never written
-// get lambda 0.
-__global__ void __launch_bounds__(256) k_guide_collect(WaveBuffers B, float* pos, float* nrm, float* dir, float* lam,
-                                                       float* lam_sum, uint32_t* n_out) {
+// objective of Mueller et al. 2019).
+__device__ __forceinline__ float record_lambda(const WaveBuffers& B, uint32_t r) {
+  const uint32_t R = B.rec_cap;
+  const float4 a = B.rec[r], b = B.rec[R + r], c = B.rec[2 * R + r], e = B.rec[3 * R + r];
+  const uint32_t slot = __float_as_uint(e.w);
+  const float4 t = B.l_term[slot], n0 = B.l_nee[2 * slot], n1 = B.l_nee[2 * slot + 1];
+  const float ar = t.x + n0.x + n1.x - e.x, ag = t.y + n0.y + n1.y - e.y, ab = t.z + n0.z + n1.z - e.z;
+  const float lr = a.w > 0.f ? ar / a.w : 0.f, lg = b.w > 0.f ? ag / b.w : 0.f, lb = c.w > 0.f ? ab / c.w : 0.f;
+  const float l = 0.2126f * lr + 0.7152f * lg + 0.0722f * lb;
+  return (l > 0.f && l < 3.0e38f) ? l : 0.f;
+}
+
+// Whether record r (lambda l) is drawn, and its weight: probability
+// p = min(1, l / tau), weight l / p = max(l, tau).
+__device__ __forceinline__ bool record_drawn(uint32_t r, float l, float tau, uint32_t seed, float& wgt) {
+  if (!(l > 0.f)) return false;
+  const float p = (tau > 0.f && l < tau) ? l / tau : 1.f;
+  uint32_t h = r * 0x9E3779B1u ^ (seed * 0x85EBCA77u + 0x27D4EB2Fu);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  h *= 0x297A2D39u;
+  h ^= h >> 15;
+  wgt = l / p;
+  return static_cast<float>(h >> 8) * (1.f / 16777216.f) < p;
+}
+
+// After a guided batch, pass 1 over its training records (grid-stride):
+// the lambda of every record into lam_all, their sum into *lam_sum_all (one
+// atomic per work-group).
+__global__ void __launch_bounds__(256) k_guide_lambda(WaveBuffers B, float* lam_all, float* lam_sum_all) {
+  __shared__ float red[4];
   const uint32_t n = min(B.counters[kRecCnt], B.rec_cap);
-  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r == 0) *n_out = n;
-  float l = 0.f;
-  if (r < n) {
-    const uint32_t R = B.rec_cap;
-    const float4 a = B.rec[r], b = B.rec[R + r], c = B.rec[2 * R + r], e = B.rec[3 * R + r];
-    const uint32_t slot = __float_as_uint(e.w);
-    const float4 t = B.l_term[slot], n0 = B.l_nee[2 * slot], n1 = B.l_nee[2 * slot + 1];
-    const float ar = t.x + n0.x + n1.x - e.x, ag = t.y + n0.y + n1.y - e.y, ab = t.z + n0.z + n1.z - e.z;
-    const float lr = a.w > 0.f ? ar / a.w : 0.f, lg = b.w > 0.f ? ag / b.w : 0.f, lb = c.w > 0.f ? ab / c.w : 0.f;
-    l = 0.2126f * lr + 0.7152f * lg + 0.0722f * lb;
-    if (!(l > 0.f) || !(l < 3.0e38f)) l = 0.f;
-    pos[3 * r] = a.x;
-    pos[3 * r + 1] = a.y;
-    pos[3 * r + 2] = a.z;
-    nrm[3 * r] = b.x;
-    nrm[3 * r + 1] = b.y;
-    nrm[3 * r + 2] = b.z;
-    dir[3 * r] = c.x;
-    dir[3 * r + 1] = c.y;
-    dir[3 * r + 2] = c.z;
-    lam[r] = l;
+  float acc = 0.f;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const float l = record_lambda(B, r);
+    lam_all[r] = l;
+    acc += l;
   }
-  const float ws = g_wave_sumf(l);
-  if (glane() == 0 && ws > 0.f) atomicAdd(lam_sum, ws);
+  const float t = block_sumf(acc, red);
+  if (threadIdx.x == 0 && t > 0.f) atomicAdd(lam_sum_all, t);
+}
+
+// Pass 2: the training batch, at most `cap` records drawn with probability
+// p = min(1, lambda / tau), tau = (sum of lambda) / target, and weighted
+// lambda / p = max(lambda, tau). The loss sum over the drawn records is then
+// an unbiased estimate of the sum over all of them, while every record with
+// lambda >= tau (the rare bright paths a guide must learn, such as
+// caustics) is kept and records with lambda 0 (no gradient) never are.
+// Each thread walks its records twice (count, then write), so a work-group
+// reserves its output range with one atomic. Drawn records go to
+// pos/nrm/dir/lam; *n_out counts them, *lam_sum adds their weights.
+__global__ void __launch_bounds__(256) k_guide_select(WaveBuffers B, const float* lam_all, const float* lam_sum_all,
+                                                      float target, uint32_t seed, float* pos, float* nrm, float* dir,
+                                                      float* lam, float* lam_sum, uint32_t* n_out, uint32_t cap) {
+  __shared__ uint32_t red[5];
+  __shared__ float redf[4];
+  __shared__ uint32_t base_sh;
+  const uint32_t n = min(B.counters[kRecCnt], B.rec_cap);
+  const float tau = *lam_sum_all / target;
+  const uint32_t r0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
+  uint32_t mine = 0;
+  for (uint32_t r = r0; r < n; r += stride) {
+    float w;
+    if (record_drawn(r, lam_all[r], tau, seed, w)) ++mine;
+  }
+  uint32_t total;
+  const uint32_t before = block_excl_scan(mine, red, &total);
+  if (threadIdx.x == 0) base_sh = total > 0u ? atomicAdd(n_out, total) : 0u;
+  __syncthreads();
+  uint32_t k = base_sh + before;
+  float added = 0.f;
+  const uint32_t R = B.rec_cap;
+  for (uint32_t r = r0; r < n && k < cap; r += stride) {
+    float w;
+    if (!record_drawn(r, lam_all[r], tau, seed, w)) continue;
+    const float4 a = B.rec[r], b = B.rec[R + r], c = B.rec[2 * R + r];
+    pos[3 * k] = a.x;
+    pos[3 * k + 1] = a.y;
+    pos[3 * k + 2] = a.z;
+    nrm[3 * k] = b.x;
+    nrm[3 * k + 1] = b.y;
+    nrm[3 * k + 2] = b.z;
+    dir[3 * k] = c.x;
+    dir[3 * k + 1] = c.y;
+    dir[3 * k + 2] = c.z;
+    lam[k] = w;
+    added += w;
+    ++k;
+  }
+  const float t = block_sumf(added, redf);
+  if (threadIdx.x == 0 && t > 0.f) atomicAdd(lam_sum, t);
 }
 
 // Rearranges the FP16 row-major weights into MFMA operand tiles.
@@ -390,15 +508,22 @@ hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, 
   return hipGetLastError();
 }
 
-hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, const GuideNet& net, int grid) {
-  hipLaunchKernelGGL(k_guide_infer, dim3(grid), dim3(64), 0, s, B, qi, net);
+hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, int depth, const GuideNet& net, int grid) {
+  // work-groups of kInferWaves waves; `grid` counts waves
+  const int blocks = (grid + kInferWaves - 1) / kInferWaves;
+  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(64 * kInferWaves), 0, s, B, qi, depth, net);
   return hipGetLastError();
 }
 
-hipError_t launch_guide_collect(hipStream_t s, const WaveBuffers& B, float* pos, float* nrm, float* dir, float* lam,
-                                float* lam_sum, uint32_t* n_out) {
+hipError_t launch_guide_collect(hipStream_t s, const WaveBuffers& B, float* lam_all, float* lam_sum_all,
+                                float target, uint32_t seed, float* pos, float* nrm, float* dir, float* lam,
+                                float* lam_sum, uint32_t* n_out, uint32_t cap) {
+  // grid-stride passes: about 16 records per thread at full capacity
   const uint32_t n = B.rec_cap > 0u ? B.rec_cap : 1u;
-  hipLaunchKernelGGL(k_guide_collect, dim3((n + 255) / 256), dim3(256), 0, s, B, pos, nrm, dir, lam, lam_sum, n_out);
+  const uint32_t blocks = std::max(1u, std::min(4096u, (n + 4095u) / 4096u));
+  hipLaunchKernelGGL(k_guide_lambda, dim3(blocks), dim3(256), 0, s, B, lam_all, lam_sum_all);
+  hipLaunchKernelGGL(k_guide_select, dim3(blocks), dim3(256), 0, s, B, lam_all, lam_sum_all, target, seed, pos, nrm,
+                     dir, lam, lam_sum, n_out, cap);
   return hipGetLastError();
 }
 

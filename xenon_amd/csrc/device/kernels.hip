@@ -283,9 +283,13 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
           const DMat m = load_mat(S.mats, sf.mat);
           const bool delta = m.delta();
           // guided vertex: one-sample MIS of the stored guide mixture and
-          // the BSDF sampler, with probabilities alpha and 1 - alpha
+          // the BSDF sampler, with probabilities alpha and 1 - alpha. At
+          // depth 0 a pixel's samples share its first sample's mixture (see
+          // k_guide_infer); an entry without a valid mixture is not guided.
           const float alpha = kGuided ? *B.alpha_now : 0.f;
-          const bool guided = kGuided && depth < B.guide_depth && lambert_exact(m);
+          const uint32_t mi = (kGuided && depth == 0) ? static_cast<uint32_t>(ck.base) + off % static_cast<uint32_t>(ck.npix) : i;
+          const bool guided = kGuided && depth < B.guide_depth && lambert_exact(m) &&
+                              B.mix[8 * B.capacity + mi].x >= 0.f;
           // area-light NEE (cpu_tiled.cpp:88-139)
           if (S.num_lights > 0 && !delta) {
             const int nl = S.num_lights;
@@ -314,7 +318,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               if (cl > 1e-6f && wi_l.z > 1e-6f) {
                 const f3 f = nee_f(wo_l, wi_l, m);
                 float bp = nee_pdf(wo_l, wi_l, m);
-                if (guided) bp = alpha * stored_pdf(B.mix, B.capacity, i, wi) + (1.f - alpha) * bp;
+                if (guided) bp = alpha * stored_pdf(B.mix, B.capacity, mi, wi) + (1.f - alpha) * bp;
                 const float lpdf = ((1.0f / l0.w) * pick) * (d2 / cl);
                 const float w = mis_power2(lpdf, bp);
                 const float cs = wi_l.z;
@@ -344,7 +348,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               if (wi_l.z > 1e-6f) {
                 const f3 f = nee_f(wo_l, wi_l, m);
                 float bp = nee_pdf(wo_l, wi_l, m);
-                if (guided) bp = alpha * stored_pdf(B.mix, B.capacity, i, sdir) + (1.f - alpha) * bp;
+                if (guided) bp = alpha * stored_pdf(B.mix, B.capacity, mi, sdir) + (1.f - alpha) * bp;
                 const float cs = wi_l.z;
                 const float lpdf = 1e8f * spick;
                 const float w = mis_power2(lpdf, bp);
@@ -372,7 +376,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
                 const float u0 = rng.f();
                 const float u1 = rng.f();
                 const float u2 = rng.f();
-                wiw = stored_sample(B.mix, B.capacity, i, u0, u1, u2);
+                wiw = stored_sample(B.mix, B.capacity, mi, u0, u1, u2);
                 wi_l = fs.local(wiw);
               } else {
                 BSample bs;
@@ -385,12 +389,13 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               // of the shading normal (the side the BSDF sampler draws on)
               const bool same = (wo_l.z < 0.f) ? (wi_l.z < 0.f) : (wi_l.z > 0.f);
               const float ci2 = fabsf(wi_l.z);
-              const float pc = alpha * stored_pdf(B.mix, B.capacity, i, wiw) + (1.f - alpha) * (same ? ci2 * kInvPiD : 0.f);
+              const float pc = alpha * stored_pdf(B.mix, B.capacity, mi, wiw) + (1.f - alpha) * (same ? ci2 * kInvPiD : 0.f);
               if (!same || ci2 < 1e-8f || !(pc >= 1e-8f)) {
                 alive = false;
               } else {
-                if (B.rec_every > 0u && record_path(slot, v.batch, B.rec_every)) {
-                  const uint32_t r = atomicAdd(&B.counters[kRecCnt], 1u);
+                const bool rec = B.rec_every > 0u && record_path(slot, v.batch, B.rec_every);
+                const uint32_t r = wave_append(&B.counters[kRecCnt], rec);   // one atomic per wave
+                if (rec) {
                   if (r < B.rec_cap) {
                     const uint32_t R = B.rec_cap;
                     const float nan = __builtin_nanf("");

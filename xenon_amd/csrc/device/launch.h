@@ -37,9 +37,13 @@ hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, 
                              size_t count, float lr, float b1, float b2, float eps, float bc1, float bc2,
                              float grad_scale, const uint32_t* n_dev, uint32_t cap, int nslices);
 hipError_t launch_tile_weights(hipStream_t s, const uint16_t* w, uint16_t* wt);
-hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, const GuideNet& net, int grid);
-hipError_t launch_guide_collect(hipStream_t s, const WaveBuffers& B, float* pos, float* nrm, float* dir, float* lam,
-                                float* lam_sum, uint32_t* n_out);
+hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, int depth, const GuideNet& net, int grid);
+// Training batch of the batch just rendered: lambda of every record, then
+// at most `cap` records drawn in proportion to lambda (target: expected
+// count), written to pos/nrm/dir/lam; *n_out and *lam_sum must be zero.
+hipError_t launch_guide_collect(hipStream_t s, const WaveBuffers& B, float* lam_all, float* lam_sum_all,
+                                float target, uint32_t seed, float* pos, float* nrm, float* dir, float* lam,
+                                float* lam_sum, uint32_t* n_out, uint32_t cap);
 hipError_t launch_to_half(hipStream_t s, const float* src, uint16_t* dst, size_t count);
 
 }  // namespace xnh

@@ -61,7 +61,9 @@ struct GuideState {
   float4* mix = nullptr;
   float4* rec = nullptr;
   float *t_pos = nullptr, *t_nrm = nullptr, *t_dir = nullptr, *t_lam = nullptr, *t_lam_sum = nullptr;
+  float *lam_all = nullptr, *lam_sum_all = nullptr;   // lambda of every record, and their sum
   uint32_t* t_n = nullptr;
+  uint32_t collects = 0;          // training batches drawn (seeds the selection)
   float* alpha_dev = nullptr;     // MIS probability of the guide for the next batch
   float alpha_host = 0.f;
   bool train = true;
@@ -465,7 +467,7 @@ int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
     HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth, c->counting));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     if (c->guide.on && d < c->B.guide_depth)
-      HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, c->guide.net, c->grid_shade * 2));
+      HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, d, c->guide.net, c->grid_shade * 2));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     HIP_OR_FAIL(c, launch_shade(s, c->B, qi, d, c->grid_shade));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
@@ -511,10 +513,15 @@ int ensure_graph(xnh_ctx* c, int depth) {
 int guide_train_batch(xnh_ctx* c) {
   GuideState& g = c->guide;
   hipStream_t s = c->stream;
-  const int R = static_cast<int>(c->B.rec_cap);
+  const int R = g.cfg.train_batch;
   const int K = g.cfg.train_steps;
   HIP_OR_FAIL(c, hipMemsetAsync(g.t_lam_sum, 0, sizeof(float), s));
-  HIP_OR_FAIL(c, launch_guide_collect(s, c->B, g.t_pos, g.t_nrm, g.t_dir, g.t_lam, g.t_lam_sum, g.t_n));
+  HIP_OR_FAIL(c, hipMemsetAsync(g.lam_sum_all, 0, sizeof(float), s));
+  HIP_OR_FAIL(c, hipMemsetAsync(g.t_n, 0, sizeof(uint32_t), s));
+  // expected count 0.9 R, so that the cap seldom clips the draw
+  HIP_OR_FAIL(c, launch_guide_collect(s, c->B, g.lam_all, g.lam_sum_all, 0.9f * static_cast<float>(R), g.collects++,
+                                      g.t_pos, g.t_nrm, g.t_dir, g.t_lam, g.t_lam_sum, g.t_n,
+                                      static_cast<uint32_t>(R)));
   const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
   const float lr = g.cfg.learning_rate;
   for (int k = 0; k < K; ++k) {
@@ -1034,8 +1041,8 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (cfg.t_log2 == 0) cfg.t_log2 = 19;
   if (cfg.learning_rate <= 0.f) cfg.learning_rate = 5e-3f;
   if (cfg.guide_fraction <= 0.f) cfg.guide_fraction = 0.5f;
-  if (cfg.train_batch <= 0) cfg.train_batch = 1 << 18;
-  if (cfg.record_paths <= 0) cfg.record_paths = 1 << 16;
+  if (cfg.train_batch <= 0) cfg.train_batch = 1 << 17;
+  if (cfg.record_paths <= 0 || cfg.record_paths > c->B.capacity) cfg.record_paths = c->B.capacity;
   if (cfg.train_steps <= 0) cfg.train_steps = 1;
   if (cfg.warmup_steps < 0) cfg.warmup_steps = 0;
   if (cfg.guide_depth <= 0) cfg.guide_depth = 3;
@@ -1059,8 +1066,13 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.loss, size_t(1));
   const size_t cap = static_cast<size_t>(c->B.capacity);
   const size_t R = static_cast<size_t>(cfg.train_batch);
+  // records: one per guided vertex of a recording path
+  const size_t RC = std::min(static_cast<size_t>(cfg.record_paths) * static_cast<size_t>(cfg.guide_depth),
+                             static_cast<size_t>(1) << 25);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.mix, size_t(kMixVecs) * cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.rec, 4 * R);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.rec, 4 * RC);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.lam_all, RC);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.lam_sum_all, size_t(1));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_pos, 3 * R);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_nrm, 3 * R);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.t_dir, 3 * R);
@@ -1077,7 +1089,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   g.net.wt = g.w_t;
   c->B.mix = g.mix;
   c->B.rec = g.rec;
-  c->B.rec_cap = static_cast<uint32_t>(R);
+  c->B.rec_cap = static_cast<uint32_t>(RC);
   c->B.rec_every = static_cast<uint32_t>(std::max<size_t>(1, cap / static_cast<size_t>(cfg.record_paths)));
   c->B.guide_alpha = cfg.guide_fraction;
   c->B.alpha_now = g.alpha_dev;
