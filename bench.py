@@ -22,12 +22,14 @@ rendering the same samples again (untimed) through the counting variants of
 the traversal kernels, which trace exactly the same rays.
 
 guided: the guided-vs-plain comparison at equal wall-clock time on the
-caustic room (a 499,280-triangle delta-glass sphere in a closed diffuse room
-lit only through a wall opening): a converged plain render (sample indices
-disjoint from the compared runs) is the reference; plain and guided renders
-each get the same wall-clock budget (guided: training after every batch
-included); relMSE = mean over finite pixels and channels of
-(x - r)^2 / (r^2 + 1e-2).
+caustic room at 1280x720 (a 499,280-triangle delta-glass sphere in a closed
+diffuse room lit only through a wall opening): a plain render of 16384
+samples (sample indices disjoint from the compared runs) is the reference,
+its own noise estimated from its two halves and subtracted; plain and guided
+renders each get the same wall-clock budget (guided: training after every
+batch included); relMSE = mean over finite pixels of the channel mean of
+(x - r)^2 / (r^2 + 1e-2), also reported without the 0.1 % largest pixel
+errors, and the median pixel error.
 
 cpu_baseline: the reference's CPU tiled integrator, compiled from its sources
 with its Release flags (oracle/_ref) where present, else the oracle's
@@ -84,8 +86,36 @@ def relmse(x, r):
     return float(np.mean((x[ok] - r[ok]) ** 2 / (r[ok] ** 2 + 1e-2)))
 
 
+def relmse_stats(x, r, ref_err=None):
+    """relMSE of image x against reference r (mean over finite pixels of the
+    channel-mean (x - r)^2 / (r^2 + 1e-2)), the same with the 0.1 % largest
+    pixel errors left out (single-path outliers dominate the mean at short
+    times), and the median pixel error. ref_err: per-pixel error the
+    reference's own noise adds (subtracted, see guided_comparison)."""
+    import numpy as np
+    e = (x - r) ** 2 / (r ** 2 + 1e-2)
+    e = e.reshape(-1, 3).mean(1)
+    if ref_err is not None:
+        e = e - ref_err
+    e = e[np.isfinite(e)]
+    s = np.sort(e)
+    keep = s[: max(1, int(len(s) * 0.999))]
+    return {"relmse": round(float(e.mean()), 6), "relmse_trim999": round(float(keep.mean()), 6),
+            "median": round(float(np.median(e)), 6)}
+
+
 def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, freq: int):
-    """Plain vs guided relMSE at equal wall-clock on the caustic room."""
+    """Plain vs guided relMSE at equal wall-clock on the caustic room.
+
+    Reference: a plain render of ref_spp samples from sample indices disjoint
+    from the compared runs, made of two halves; its own noise adds about a
+    quarter of the halves' mutual error to every measured pixel error (the
+    difference of two halves has four times the variance of the full
+    reference), and that estimate is subtracted. The guide uses the
+    library's defaults (16-level hash grid of 2^19 entries x 2 FP16
+    features, 3x64 MLP, 8 vMF lobes, training on up to 2^18 samples after
+    every batch); the guided run's time includes its training."""
+    import numpy as np
     from xenon_amd import HipRenderer, HipScene, scenes
 
     desc = scenes.caustic_room(width=width, height=height, freq=freq)
@@ -93,21 +123,28 @@ def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, fre
     r = HipRenderer(desc.width, desc.height, desc.max_bounces, desc.min_bounces)
     r.upload_scene(hs)
     cam = hs.camera()
-    chunk = 64
     t0 = time.perf_counter()
-    done = 0
-    while done < ref_spp:
-        n = min(chunk * 16, ref_spp - done)
-        r.render(cam, 10_000_001 + done, n, want_image=False)
-        done += n
-    ref, _ = r.read_accum()
-    ref = ref / ref_spp
+    halves = []
+    for h in range(2):
+        r.reset()
+        done = 0
+        half = ref_spp // 2
+        while done < half:
+            n = min(256, half - done)
+            r.render(cam, 10_000_001 + h * half + done, n, want_image=False)
+            done += n
+        acc, _ = r.read_accum()
+        halves.append(acc / half)
+    ref = 0.5 * (halves[0] + halves[1])
+    d = (halves[0] - halves[1]) ** 2 / (ref ** 2 + 1e-2)
+    ref_err = 0.25 * d.reshape(-1, 3).mean(1)
     log(f"guided: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
 
     def run(guided: bool):
         r.reset()
         if guided:
             r.guide_enable()
+        chunk = max(r.tuning()["spp_per_launch"], 1)
         t = time.perf_counter()
         spp = 0
         while time.perf_counter() - t < seconds:
@@ -121,14 +158,20 @@ def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, fre
 
     plain, sp, tp = run(False)
     guided, sg, tg = run(True)
-    rp, rg = relmse(plain, ref), relmse(guided, ref)
-    log(f"guided: plain {sp} spp relMSE {rp:.5f}; guided {sg} spp relMSE {rg:.5f}")
+    ep, eg = relmse_stats(plain, ref, ref_err), relmse_stats(guided, ref, ref_err)
+    log(f"guided: plain {sp} spp {ep}; guided {sg} spp {eg}")
     r.close()
+
+    def ratio(k):
+        return round(ep[k] / eg[k], 3) if eg[k] > 0 else None
+
     return {"workload": f"caustic room ({desc.num_tris} tris), {desc.width}x{desc.height}, depth {desc.max_bounces}",
             "seconds": round(seconds, 2), "reference_spp": ref_spp,
-            "plain": {"spp": sp, "seconds": round(tp, 2), "relmse": round(rp, 6)},
-            "guided": {"spp": sg, "seconds": round(tg, 2), "relmse": round(rg, 6)},
-            "relmse_ratio_plain_over_guided": round(rp / rg, 3) if rg > 0 else None}
+            "plain": {"spp": sp, "seconds": round(tp, 2), **ep},
+            "guided": {"spp": sg, "seconds": round(tg, 2), **eg},
+            "relmse_ratio_plain_over_guided": ratio("relmse"),
+            "trim999_ratio_plain_over_guided": ratio("relmse_trim999"),
+            "median_ratio_plain_over_guided": ratio("median")}
 
 
 def cpu_baseline(desc, budget_s: float):
@@ -172,11 +215,11 @@ def main():
     ap.add_argument("--lds-stack", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--guided-seconds", type=float, default=1.5,
+    ap.add_argument("--guided-seconds", type=float, default=8.0,
                     help="wall-clock budget of each side of the guided-vs-plain comparison (0: skip)")
-    ap.add_argument("--guided-width", type=int, default=320)
-    ap.add_argument("--guided-height", type=int, default=180)
-    ap.add_argument("--guided-ref-spp", type=int, default=65536)
+    ap.add_argument("--guided-width", type=int, default=1280)
+    ap.add_argument("--guided-height", type=int, default=720)
+    ap.add_argument("--guided-ref-spp", type=int, default=16384)
     ap.add_argument("--guided-freq", type=int, default=158)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device-to-device copy of MIB MiB five times (the reference kernel "

@@ -231,7 +231,7 @@ typedef struct {
   float guide_fraction;    /* one-sample MIS: probability of sampling the guide (0: 0.5) */
   uint32_t seed;           /* parameter initialisation */
   float bounds_lo[3], bounds_hi[3];   /* scene box mapped to the unit cube */
-  int32_t train_batch;     /* training samples per batch (0: 2^17) */
+  int32_t train_batch;     /* training samples per batch (0: 2^18) */
   int32_t record_paths;    /* paths per batch that record training samples (0: all) */
   int32_t train_steps;     /* optimisation steps per batch, each on an equal slice of its records (0: 1) */
   int32_t warmup_steps;    /* the guide's MIS probability ramps from 0 to guide_fraction over this many steps */

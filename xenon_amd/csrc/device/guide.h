@@ -91,11 +91,20 @@ __device__ __forceinline__ void sh16(f3 n, float* o) {
   o[15] = 0.59004358992664352f * x * (-xx + 3.0f * yy);
 }
 
-// Writes the 48 FP16 encoding values of (p, n) to `row`. The corner
-// fetches of four levels at a time are issued before any is used, so a wave
-// waits on memory four times per point rather than sixteen; each level still
-// sums its corners in order 0..7.
-__device__ __forceinline__ void guide_encode(const GuideNet& net, f3 p, f3 n, _Float16* row) {
+// The 48 FP16 encoding values of (p, n), packed two per word (the lower
+// address in the low half): words 0..15 the 16 levels' two features, 16..23
+// the 16 SH coefficients. The corner fetches of four levels at a time are
+// issued before any is used, so a wave waits on memory four times per point
+// rather than sixteen; each level still sums its corners in order 0..7.
+__device__ __forceinline__ uint32_t pack_h2(float a, float b) {
+  const _Float16 ha = static_cast<_Float16>(a), hb = static_cast<_Float16>(b);
+  uint16_t ua, ub;
+  __builtin_memcpy(&ua, &ha, 2);
+  __builtin_memcpy(&ub, &hb, 2);
+  return static_cast<uint32_t>(ua) | (static_cast<uint32_t>(ub) << 16);
+}
+
+__device__ __forceinline__ void guide_encode_packed(const GuideNet& net, f3 p, f3 n, uint32_t (&o)[24]) {
   const f3 q = guide_unit_pos(net, p);
   const size_t T = static_cast<size_t>(1) << net.t_log2;
   const uint32_t* g32 = reinterpret_cast<const uint32_t*>(net.grid);   // the two features of a slot
@@ -121,13 +130,22 @@ __device__ __forceinline__ void guide_encode(const GuideNet& net, f3 p, f3 n, _F
         a0 = a0 + w[8 * lk + c] * static_cast<float>(h0);
         a1 = a1 + w[8 * lk + c] * static_cast<float>(h1);
       }
-      row[2 * (l0 + lk)] = static_cast<_Float16>(a0);
-      row[2 * (l0 + lk) + 1] = static_cast<_Float16>(a1);
+      o[l0 + lk] = pack_h2(a0, a1);
     }
   }
   float s[16];
   sh16(n, s);
-  for (int k = 0; k < 16; ++k) row[32 + k] = static_cast<_Float16>(s[k]);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o[16 + k] = pack_h2(s[2 * k], s[2 * k + 1]);
+}
+
+// The same 48 values written to `row` (kGIn FP16).
+__device__ __forceinline__ void guide_encode(const GuideNet& net, f3 p, f3 n, _Float16* row) {
+  uint32_t o[24];
+  guide_encode_packed(net, p, n, o);
+  uint32_t* r32 = reinterpret_cast<uint32_t*>(row);
+#pragma unroll
+  for (int k = 0; k < 24; ++k) r32[k] = o[k];
 }
 
 // LDS written by some lanes of a wave and read by others: the wave's LDS
@@ -218,13 +236,12 @@ __device__ __forceinline__ void stage_weights(const uint16_t* wt, h4v* wl) {
   __syncthreads();
 }
 
-// wave: forward pass of 64 points, one per lane (p: world position, n:
-// normal). act: this wave's 64 x kGIn FP16 LDS rows; wl: the weights in LDS
+// wave: the MLP on the 64 encoded rows in act (64 x kGIn FP16 LDS rows of
+// this wave, written by all its lanes); wl: the weights in LDS
 // (stage_weights). On return the lane's 48 raw outputs (FP32 accumulation,
 // stored FP16) are at act + lane kGIn.
-__device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3 n, _Float16* act, const h4v* wl) {
+__device__ __forceinline__ void guide_mlp_wave(_Float16* act, const h4v* wl) {
   const int lane = static_cast<int>(threadIdx.x & 63u), g = lane >> 4, li = lane & 15;
-  guide_encode(net, p, n, act + lane * kGIn);
   wave_lds_sync();
   h4v x[4][3];
   for (int nt = 0; nt < 4; ++nt)
@@ -244,6 +261,13 @@ __device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3
     }
   }
   wave_lds_sync();
+}
+
+// wave: forward pass of 64 points, one per lane (p: world position, n:
+// normal); act and wl as for guide_mlp_wave.
+__device__ __forceinline__ void guide_forward_wave(const GuideNet& net, f3 p, f3 n, _Float16* act, const h4v* wl) {
+  guide_encode(net, p, n, act + (threadIdx.x & 63u) * kGIn);
+  guide_mlp_wave(act, wl);
 }
 
 // wave: the full forward pass; lds_x holds the encodings (64 x 48).

@@ -107,7 +107,9 @@ __device__ __forceinline__ float loss_grad(const float* raw, f3 w, float lam, fl
   return -lam * logf(q);
 }
 
-// wave: dW (K x N, global FP32) += A^T (K x 64) * G (64 x N); A: 64 x K rows.
+// wave: dW (K x N, global FP32) = A^T (K x 64) * G (64 x N); A: 64 x K rows.
+// dW is this work-group's own slice of the partial-gradient buffer (summed
+// over work-groups by k_wgrad_reduce), so plain stores suffice.
 __device__ __forceinline__ void weight_grad(const _Float16* A, int K, const _Float16* G, int N, float* dW) {
   const int lane = glane();
   for (int rt = 0; rt < K / 16; ++rt)
@@ -115,7 +117,7 @@ __device__ __forceinline__ void weight_grad(const _Float16* A, int K, const _Flo
       const f4v acc = mma_tile<true, false>(A, K, 16 * rt, G, N, 16 * ct, 64);
       for (int j = 0; j < 4; ++j) {
         const int r = 16 * rt + 4 * (lane >> 4) + j, col = 16 * ct + (lane & 15);
-        if (acc[j] != 0.f) atomicAdd(&dW[r * N + col], acc[j]);
+        dW[r * N + col] = acc[j];
       }
     }
 }
@@ -205,17 +207,77 @@ __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const f
 // normal input is the geometric normal) and stores the mixture for k_shade.
 // One wave per block, grid-stride over the batch's 64-entry units. Launched
 // for the bounces that are guided (below the configured guide depth).
-// Guide mixtures for the hits of queue qi (after its closest-hit pass): one
-// stored mixture per entry whose hit is a non-emitting exact-Lambertian
-// surface; every other entry gets an invalid mark (a negative first lobe
-// weight), which the shading kernel checks. At depth 0 the queue is in
-// camera-ray order (entry = sample * npix + pixel within a chunk) and only
-// each pixel's first sample of the batch is evaluated: the pixel's other
-// samples use that mixture as their guide too (a proposal that does not
-// depend on their own random numbers, so the estimate stays unbiased).
-constexpr int kInferWaves = 4;   // waves per work-group of k_guide_infer
+// Guide mixtures for the hits of queue qi (after its closest-hit pass), in
+// two kernels: k_guide_encode gathers the hash-grid encoding of every entry
+// whose hit is a non-emitting exact-Lambertian surface (memory-latency
+// bound, so it runs lean and at high occupancy) into feat, and k_guide_mlp
+// runs the MLP on those rows from LDS-resident weights and stores the
+// mixture (MFMA bound). Every entry gets a mark in the first lobe weight:
+// negative = no mixture, which the shading kernel checks. At depth 0 the
+// queue is in camera-ray order (entry = sample * npix + pixel within a
+// chunk) and only each pixel's first sample of the batch is evaluated: the
+// pixel's other samples use that mixture as their guide too (a proposal that
+// does not depend on their own random numbers, so the estimate stays
+// unbiased). feat: 6 uint4 per entry, feat[k * C + i].
+constexpr int kInferWaves = 4;   // waves per work-group of both kernels
 
-__global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(WaveBuffers B, int qi, int depth, GuideNet net) {
+// The entries of one 64-entry unit that need a mixture: false for units
+// past the last evaluated entry (returns false for the whole unit).
+struct InferUnit {
+  uint32_t i, off, lim;
+  const DevChunk* ck;
+};
+__device__ __forceinline__ bool infer_unit(const BatchView& v, uint32_t unit, int depth, InferUnit& u) {
+  const int ci = uniform(find_chunk(v, unit));
+  u.ck = &v.ch[ci];
+  const uint32_t first = (unit - v.prefix[ci]) * 64u;
+  u.lim = depth == 0 ? min(v.cnt[ci], static_cast<uint32_t>(u.ck->npix)) : v.cnt[ci];
+  if (first >= u.lim) return false;   // uniform: a later sample of every pixel in the unit
+  u.off = first + static_cast<uint32_t>(glane());
+  u.i = static_cast<uint32_t>(u.ck->base) + u.off;
+  return true;
+}
+
+__global__ void __launch_bounds__(64 * kInferWaves) k_guide_encode(WaveBuffers B, int qi, int depth, GuideNet net,
+                                                                   uint4* feat) {
+  __shared__ BatchView v;
+  const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
+  load_batch(v, B, counts, false);
+  const uint32_t total = v.prefix[v.nch];
+  const PathQueue q = B.q[qi];
+  const uint32_t C = static_cast<uint32_t>(B.capacity);
+  const uint32_t wid = threadIdx.x >> 6;
+  for (uint32_t unit = blockIdx.x * kInferWaves + wid; unit < total; unit += gridDim.x * kInferWaves) {
+    InferUnit u;
+    if (!infer_unit(v, unit, depth, u)) continue;
+    if (u.off >= u.lim) continue;
+    const DevScene& S = B.jobs[uniform(u.ck->job)].S;
+    bool want = false;
+    f3 p{0.f, 0.f, 0.f}, n{0.f, 0.f, 1.f};
+    const float4 hv = B.hit[u.i];
+    const int inst = __float_as_int(hv.w);
+    if (inst >= 0) {
+      const float4 r0 = q.ray0[u.i], r1 = q.ray1[u.i];
+      const HitRef h{hv.x, hv.y, hv.z, B.hit_slot[u.i], inst};
+      const GeoSurface sf = resolve_geo(S, h, f3{r0.x, r0.y, r0.z}, f3{r0.w, r1.x, r1.y});
+      const bool emitter = sf.lmap >= 0 && S.lmap[sf.lmap + static_cast<int>(sf.tri)] >= 0;
+      if (!emitter && lambert_exact(load_mat(S.mats, sf.mat))) {
+        want = true;
+        p = sf.pos;
+        n = sf.gn;
+      }
+    }
+    B.mix[8 * C + u.i] = make_float4(want ? 1.f : -1.f, 0.f, 0.f, 0.f);
+    if (!want) continue;
+    uint32_t o[24];
+    guide_encode_packed(net, p, n, o);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) feat[k * C + u.i] = make_uint4(o[4 * k], o[4 * k + 1], o[4 * k + 2], o[4 * k + 3]);
+  }
+}
+
+__global__ void __launch_bounds__(64 * kInferWaves) k_guide_mlp(WaveBuffers B, int qi, int depth, GuideNet net,
+                                                                const uint4* feat) {
   __shared__ BatchView v;
   __shared__ _Float16 act[kInferWaves][64 * kGIn];
   __shared__ h4v wl[kWTiles];
@@ -223,40 +285,20 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(WaveBuffers B,
   load_batch(v, B, counts, false);
   stage_weights(net.wt, wl);
   const uint32_t total = v.prefix[v.nch];
-  const PathQueue q = B.q[qi];
   const uint32_t C = static_cast<uint32_t>(B.capacity);
   const uint32_t wid = threadIdx.x >> 6;
+  _Float16* a = act[wid];
+  uint4* row = reinterpret_cast<uint4*>(a + glane() * kGIn);
   // waves work independently from here on (no work-group barrier below)
   for (uint32_t unit = blockIdx.x * kInferWaves + wid; unit < total; unit += gridDim.x * kInferWaves) {
-    const int ci = uniform(find_chunk(v, unit));
-    const DevChunk& ck = v.ch[ci];
-    const uint32_t first = (unit - v.prefix[ci]) * 64u;
-    const uint32_t lim = depth == 0 ? min(v.cnt[ci], static_cast<uint32_t>(ck.npix)) : v.cnt[ci];
-    if (first >= lim) continue;   // uniform: a later sample of every pixel in the unit
-    const DevScene& S = B.jobs[uniform(ck.job)].S;
-    const uint32_t off = first + static_cast<uint32_t>(glane());
-    const uint32_t i = static_cast<uint32_t>(ck.base) + off;
-    bool want = false;
-    f3 p{net.lo[0], net.lo[1], net.lo[2]}, n{0.f, 0.f, 1.f};
-    if (off < lim) {
-      const float4 hv = B.hit[i];
-      const int inst = __float_as_int(hv.w);
-      if (inst >= 0) {
-        const float4 r0 = q.ray0[i], r1 = q.ray1[i];
-        const HitRef h{hv.x, hv.y, hv.z, B.hit_slot[i], inst};
-        const GeoSurface sf = resolve_geo(S, h, f3{r0.x, r0.y, r0.z}, f3{r0.w, r1.x, r1.y});
-        const bool emitter = sf.lmap >= 0 && S.lmap[sf.lmap + static_cast<int>(sf.tri)] >= 0;
-        if (!emitter && lambert_exact(load_mat(S.mats, sf.mat))) {
-          want = true;
-          p = sf.pos;
-          n = sf.gn;
-        }
-      }
-      if (!want) B.mix[8 * C + i] = make_float4(-1.f, 0.f, 0.f, 0.f);
-    }
+    InferUnit u;
+    if (!infer_unit(v, unit, depth, u)) continue;
+    const bool want = u.off < u.lim && B.mix[8 * C + u.i].x > 0.f;
     if (__ballot(want) == 0ull) continue;
-    guide_forward_wave(net, p, n, act[wid], wl);
-    if (want) decode_store_mixture(act[wid] + glane() * kGIn, B.mix, C, i);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) row[k] = want ? feat[k * C + u.i] : make_uint4(0u, 0u, 0u, 0u);
+    guide_mlp_wave(a, wl);
+    if (want) decode_store_mixture(a + glane() * kGIn, B.mix, C, u.i);
   }
 }
 
@@ -385,8 +427,9 @@ __global__ void __launch_bounds__(256) k_tile_weights(const uint16_t* w, uint16_
 __global__ void __launch_bounds__(64) k_guide_train(GuideNet net, int n, const float* pos, const float* nrm,
                                                     const float* dir, const float* lam, float lam_scale,
                                                     const uint32_t* n_dev, const float* lam_sum_dev, int slice,
-                                                    int nslices, float* grid_g, float* w_g, float* loss_sum) {
+                                                    int nslices, float* grid_g, float* w_part, float* loss_sum) {
   __shared__ TrainLds s;
+  float* w_g = w_part + static_cast<size_t>(blockIdx.x) * kWTotal;   // this work-group's weight gradients
   int begin = 0;
   if (n_dev) {
     const uint32_t cnt = min(*n_dev, static_cast<uint32_t>(n));
@@ -443,6 +486,30 @@ __global__ void __launch_bounds__(64) k_guide_train(GuideNet net, int n, const f
   }
 }
 
+// Sums the per-work-group weight gradients of a k_guide_train launch into
+// w_g: thread (w, chunk) adds partials of work-groups chunk*kRedChunk ..
+// +kRedChunk for weight w, one atomic per (w, chunk). The number of
+// work-groups that ran follows from the slice's record count, as in
+// k_guide_train.
+constexpr int kRedChunk = 64;
+__global__ void __launch_bounds__(256) k_wgrad_reduce(const float* w_part, float* w_g, int n, const uint32_t* n_dev,
+                                                      int slice, int nslices) {
+  const int w = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (w >= kWTotal) return;
+  int begin = 0;
+  if (n_dev) {
+    const uint32_t cnt = min(*n_dev, static_cast<uint32_t>(n));
+    begin = static_cast<int>((static_cast<uint64_t>(cnt) * slice) / nslices);
+    n = static_cast<int>((static_cast<uint64_t>(cnt) * (slice + 1)) / nslices);
+  }
+  const int used = n > begin ? (n - begin + 63) / 64 : 0;
+  const int b0 = static_cast<int>(blockIdx.y) * kRedChunk, b1 = min(used, b0 + kRedChunk);
+  if (b0 >= b1) return;
+  float acc = 0.f;
+  for (int b = b0; b < b1; ++b) acc += w_part[static_cast<size_t>(b) * kWTotal + w];
+  if (acc != 0.f) atomicAdd(&w_g[w], acc);
+}
+
 // Adam with bias correction on FP32 masters; refreshes the FP16 copies and
 // clears the gradients. grad_scale converts summed gradients to means.
 // With n_dev non-null the gradient scale is nslices / min(*n_dev, cap), one
@@ -488,14 +555,21 @@ hipError_t launch_guide_query(hipStream_t s, const GuideNet& net, int n, const f
   return hipGetLastError();
 }
 
+int guide_train_blocks(int n, bool device_count, int nslices) {
+  const int per = device_count ? (n + nslices - 1) / nslices + 1 : n;
+  return (per + 63) / 64;
+}
+
 hipError_t launch_guide_train(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
                               const float* dir, const float* lam, float lam_scale, const uint32_t* n_dev,
-                              const float* lam_sum_dev, int slice, int nslices, float* grid_g, float* w_g,
-                              float* loss_sum) {
+                              const float* lam_sum_dev, int slice, int nslices, float* grid_g, float* w_part,
+                              float* w_g, float* loss_sum) {
   if (n <= 0) return hipSuccess;
-  const int per = n_dev ? (n + nslices - 1) / nslices + 1 : n;
-  hipLaunchKernelGGL(k_guide_train, dim3((per + 63) / 64), dim3(64), 0, s, net, n, pos, nrm, dir, lam, lam_scale,
-                     n_dev, lam_sum_dev, slice, nslices, grid_g, w_g, loss_sum);
+  const int blocks = guide_train_blocks(n, n_dev != nullptr, nslices);
+  hipLaunchKernelGGL(k_guide_train, dim3(blocks), dim3(64), 0, s, net, n, pos, nrm, dir, lam, lam_scale, n_dev,
+                     lam_sum_dev, slice, nslices, grid_g, w_part, loss_sum);
+  const dim3 rgrid((kWTotal + 255) / 256, (blocks + kRedChunk - 1) / kRedChunk);
+  hipLaunchKernelGGL(k_wgrad_reduce, rgrid, dim3(256), 0, s, w_part, w_g, n, n_dev, slice, nslices);
   return hipGetLastError();
 }
 
@@ -508,10 +582,13 @@ hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, 
   return hipGetLastError();
 }
 
-hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, int depth, const GuideNet& net, int grid) {
+hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, int depth, const GuideNet& net, uint4* feat,
+                              int grid) {
   // work-groups of kInferWaves waves; `grid` counts waves
   const int blocks = (grid + kInferWaves - 1) / kInferWaves;
-  hipLaunchKernelGGL(k_guide_infer, dim3(blocks), dim3(64 * kInferWaves), 0, s, B, qi, depth, net);
+  hipLaunchKernelGGL(k_guide_encode, dim3(blocks), dim3(64 * kInferWaves), 0, s, B, qi, depth, net, feat);
+  hipLaunchKernelGGL(k_guide_mlp, dim3(blocks), dim3(64 * kInferWaves), 0, s, B, qi, depth, net,
+                     static_cast<const uint4*>(feat));
   return hipGetLastError();
 }
 

@@ -29,15 +29,23 @@ hipError_t launch_guide_eval(hipStream_t s, const GuideNet& net, int n, const fl
                              float* raw);
 hipError_t launch_guide_query(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
                               const float* u, float* dir, float* pdf);
+// Work-groups of one launch_guide_train call (each needs kWTotal floats of
+// w_part): n records, or with device counts the largest slice of n.
+int guide_train_blocks(int n, bool device_count, int nslices);
+// Forward/backward over the records; grid gradients are added to grid_g,
+// weight gradients go through w_part (per work-group) and are added to w_g.
 hipError_t launch_guide_train(hipStream_t s, const GuideNet& net, int n, const float* pos, const float* nrm,
                               const float* dir, const float* lam, float lam_scale, const uint32_t* n_dev,
-                              const float* lam_sum_dev, int slice, int nslices, float* grid_g, float* w_g,
-                              float* loss_sum);
+                              const float* lam_sum_dev, int slice, int nslices, float* grid_g, float* w_part,
+                              float* w_g, float* loss_sum);
 hipError_t launch_guide_adam(hipStream_t s, float* master, uint16_t* half_copy, float* grad, float* m, float* v,
                              size_t count, float lr, float b1, float b2, float eps, float bc1, float bc2,
                              float grad_scale, const uint32_t* n_dev, uint32_t cap, int nslices);
 hipError_t launch_tile_weights(hipStream_t s, const uint16_t* w, uint16_t* wt);
-hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, int depth, const GuideNet& net, int grid);
+// Guide mixtures for the hits of queue qi (encode, then MLP); feat: 6 uint4
+// per queue entry of scratch.
+hipError_t launch_guide_infer(hipStream_t s, const WaveBuffers& B, int qi, int depth, const GuideNet& net, uint4* feat,
+                              int grid);
 // Training batch of the batch just rendered: lambda of every record, then
 // at most `cap` records drawn in proportion to lambda (target: expected
 // count), written to pos/nrm/dir/lam; *n_out and *lam_sum must be zero.

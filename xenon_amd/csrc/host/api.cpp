@@ -59,9 +59,12 @@ struct GuideState {
   // guided rendering: per-entry mixtures, training records and the device
   // training batch built from them
   float4* mix = nullptr;
+  uint4* feat = nullptr;          // hash-grid encodings of a queue (6 uint4 per entry)
   float4* rec = nullptr;
   float *t_pos = nullptr, *t_nrm = nullptr, *t_dir = nullptr, *t_lam = nullptr, *t_lam_sum = nullptr;
   float *lam_all = nullptr, *lam_sum_all = nullptr;   // lambda of every record, and their sum
+  float* w_part = nullptr;        // per-work-group weight gradients of a training launch
+  int w_part_blocks = 0;
   uint32_t* t_n = nullptr;
   uint32_t collects = 0;          // training batches drawn (seeds the selection)
   float* alpha_dev = nullptr;     // MIS probability of the guide for the next batch
@@ -467,7 +470,7 @@ int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
     HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth, c->counting));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     if (c->guide.on && d < c->B.guide_depth)
-      HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, d, c->guide.net, c->grid_shade * 2));
+      HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, d, c->guide.net, c->guide.feat, c->grid_shade * 2));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     HIP_OR_FAIL(c, launch_shade(s, c->B, qi, d, c->grid_shade));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
@@ -506,6 +509,25 @@ int ensure_graph(xnh_ctx* c, int depth) {
   return XNH_OK;
 }
 
+// Room for the per-work-group weight gradients of a training launch of
+// `blocks` work-groups.
+static int ensure_wpart(xnh_ctx* c, int blocks) {
+  GuideState& g = c->guide;
+  if (blocks <= g.w_part_blocks) return XNH_OK;
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  if (g.w_part) {
+    for (auto& p : g.allocs)
+      if (p == g.w_part) p = nullptr;
+    HIP_OR_FAIL(c, hipFree(g.w_part));
+    g.w_part = nullptr;
+    g.w_part_blocks = 0;
+  }
+  const int rc = dev_alloc(c, g.allocs, &g.w_part, static_cast<size_t>(blocks) * kWTotal);
+  if (rc != XNH_OK) return rc;
+  g.w_part_blocks = blocks;
+  return XNH_OK;
+}
+
 // Guiding steps on the batch just rendered: records -> training batch (all
 // counts stay on the device) -> `train_steps` fused forward/backward + Adam
 // steps on equal slices of it; then the guide's MIS probability for the
@@ -524,10 +546,14 @@ int guide_train_batch(xnh_ctx* c) {
                                       static_cast<uint32_t>(R)));
   const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
   const float lr = g.cfg.learning_rate;
+  {
+    const int rc = ensure_wpart(c, guide_train_blocks(R, true, K));
+    if (rc != XNH_OK) return rc;
+  }
   for (int k = 0; k < K; ++k) {
     HIP_OR_FAIL(c, hipMemsetAsync(g.loss, 0, sizeof(float), s));
     HIP_OR_FAIL(c, launch_guide_train(s, g.net, R, g.t_pos, g.t_nrm, g.t_dir, g.t_lam, 1.f, g.t_n, g.t_lam_sum, k, K,
-                                      g.grid_g, g.w_g, g.loss));
+                                      g.grid_g, g.w_part, g.w_g, g.loss));
     g.step++;
     const float bc1 = 1.f - static_cast<float>(std::pow(double(b1), g.step));
     const float bc2 = 1.f - static_cast<float>(std::pow(double(b2), g.step));
@@ -1041,7 +1067,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (cfg.t_log2 == 0) cfg.t_log2 = 19;
   if (cfg.learning_rate <= 0.f) cfg.learning_rate = 5e-3f;
   if (cfg.guide_fraction <= 0.f) cfg.guide_fraction = 0.5f;
-  if (cfg.train_batch <= 0) cfg.train_batch = 1 << 17;
+  if (cfg.train_batch <= 0) cfg.train_batch = 1 << 18;
   if (cfg.record_paths <= 0 || cfg.record_paths > c->B.capacity) cfg.record_paths = c->B.capacity;
   if (cfg.train_steps <= 0) cfg.train_steps = 1;
   if (cfg.warmup_steps < 0) cfg.warmup_steps = 0;
@@ -1070,6 +1096,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   const size_t RC = std::min(static_cast<size_t>(cfg.record_paths) * static_cast<size_t>(cfg.guide_depth),
                              static_cast<size_t>(1) << 25);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.mix, size_t(kMixVecs) * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.feat, size_t(6) * cap);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.rec, 4 * RC);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.lam_all, RC);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.lam_sum_all, size_t(1));
@@ -1205,8 +1232,12 @@ static int guide_step(xnh_ctx* c, int n, const float* dp, const float* dn, const
   GuideState& g = c->guide;
   hipStream_t s = c->stream;
   HIP_OR_FAIL(c, hipMemsetAsync(g.loss, 0, 4, s));
-  HIP_OR_FAIL(c, launch_guide_train(s, g.net, n, dp, dn, dd, dl, lam_scale, nullptr, nullptr, 0, 1, g.grid_g, g.w_g,
-                                    g.loss));
+  {
+    const int rc = ensure_wpart(c, guide_train_blocks(n, false, 1));
+    if (rc != XNH_OK) return rc;
+  }
+  HIP_OR_FAIL(c, launch_guide_train(s, g.net, n, dp, dn, dd, dl, lam_scale, nullptr, nullptr, 0, 1, g.grid_g,
+                                    g.w_part, g.w_g, g.loss));
   if (apply) {
     g.step++;
     const float b1 = 0.9f, b2 = 0.99f, eps = 1e-8f;
