@@ -1071,7 +1071,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (cfg.record_paths <= 0 || cfg.record_paths > c->B.capacity) cfg.record_paths = c->B.capacity;
   if (cfg.train_steps <= 0) cfg.train_steps = 1;
   if (cfg.warmup_steps < 0) cfg.warmup_steps = 0;
-  if (cfg.guide_depth <= 0) cfg.guide_depth = 3;
+  if (cfg.guide_depth <= 0) cfg.guide_depth = 1;
   if (cfg.t_log2 < 8 || cfg.t_log2 > 24 || cfg.guide_fraction >= 1.f) return fail(c, XNH_E_INVALID, "guide config");
   xnh_guide_disable(c);
   GuideState& g = c->guide;
