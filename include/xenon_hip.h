@@ -83,7 +83,9 @@ typedef struct {
 } xnh_mesh;
 
 /* Instance (geometry/tlas.h:76-87): column-major 3x4 matrices as in
- * AffineTransform (geometry/transform.h:24-62). */
+ * AffineTransform (geometry/transform.h:24-62). mesh XNH_PRIM_GEOMETRY: the
+ * scene's analytic primitives (its PrimBLAS, GeomHandle::from_prim). */
+#define XNH_PRIM_GEOMETRY (-1)
 typedef struct {
   int32_t mesh;
   int32_t instance_id;
@@ -127,6 +129,22 @@ typedef struct {
   float sharpness, intensity;
 } xnh_sky;
 
+/* Analytic primitive (geometry/primitives.h:54-318), p by type:
+ *   XNH_PRIM_SPHERE  center xyz, radius
+ *   XNH_PRIM_PLANE   unit normal xyz, offset (dot(normal, point))
+ *   XNH_PRIM_BOX     center xyz, half-extents xyz
+ *   XNH_PRIM_DISK    center xyz, unit normal xyz, radius
+ *   XNH_PRIM_QUAD    corner xyz, edge u xyz, edge v xyz */
+#define XNH_PRIM_SPHERE 0
+#define XNH_PRIM_PLANE 1
+#define XNH_PRIM_BOX 2
+#define XNH_PRIM_DISK 3
+#define XNH_PRIM_QUAD 4
+typedef struct {
+  int32_t type, mat_id;
+  float p[10];
+} xnh_prim;
+
 /* Everything upload_scene reads from the reference's Scene
  * (scene/scene.h:52-75) after build_acceleration. */
 typedef struct {
@@ -144,6 +162,14 @@ typedef struct {
   int32_t num_dir_lights;
   const xnh_dir_light* dir_lights;
   xnh_sky sky;
+  /* Scene::all_prims and its PrimBLAS (geometry/prim_blas.h:54-110):
+   * nodes as xnh_bvh_node, prim_order = PrimBLAS::prims_ (leaf slot ->
+   * primitive index). Zero / NULL: no primitives. */
+  int32_t num_prims;
+  const xnh_prim* prims;
+  int32_t num_prim_nodes;
+  const xnh_bvh_node* prim_nodes;
+  const uint32_t* prim_order;
 } xnh_flat_scene;
 
 typedef struct {
@@ -328,6 +354,12 @@ int xnh_scene_add_light(xnh_scene* s, int32_t mesh, int32_t tri, const float* em
 int xnh_scene_add_dir_light(xnh_scene* s, const float* dir3, const float* color3, float intensity);
 int xnh_scene_set_sky(xnh_scene* s, const float* zenith3, const float* horizon3, const float* ground3,
                       float intensity, float sharpness);
+/* Analytic primitive (xnh_prim types); returns its index (the hit record's
+ * prim_id). Plane and disk normals are used as given (the scene parser
+ * normalises them, scene_file.cpp:197-222). All primitives go into one
+ * PrimBLAS, added to the TLAS as one identity instance after the instances
+ * of the meshes (Scene::build_acceleration, scene/scene.h:97-110). */
+int xnh_scene_add_primitive(xnh_scene* s, int32_t type, const float* params, int32_t mat_id);
 /* TRS instance (geometry/transform.h:105-148); none -> identity per mesh. */
 int xnh_scene_add_instance(xnh_scene* s, int32_t mesh, const float* t3, float scale, const float* rot_deg3,
                            int32_t instance_id);
@@ -345,10 +377,11 @@ int xnh_camera_look_at(const float* eye3, const float* target3, const float* up3
  * and render settings. Relative paths resolve as the reference resolves them:
  * a matfile name as "materials/<name>" under the working directory, else
  * beside the scene file; a mesh path as given.
- * Analytic primitives (sphere, box, disk, quad, plane) are not traced by
- * this backend: they are counted in skipped_primitives, not loaded. Files
- * that cannot be opened are counted in missing_files and skipped (a .mat
- * falls back to the default material), as the reference skips them.
+ * Analytic primitives (sphere, box, disk, quad, plane) are loaded as the
+ * parser does (an unknown material name: material 0, counted in
+ * unknown_materials). Files that cannot be opened are counted in
+ * missing_files and skipped (a .mat falls back to the default material), as
+ * the reference skips them.
  * One directive is this backend's own (the reference's parser skips it):
  *   instance <mesh> <tx> <ty> <tz> <scale> <rx> <ry> <rz> [id]
  * places mesh number <mesh> (file order) with the TRS of
@@ -360,9 +393,10 @@ typedef struct {
   int32_t width, height, samples;       /* SceneConfig (scene/scene_file.h:11-19) */
   int32_t min_bounces, max_bounces;
   int32_t meshes;                       /* meshes loaded */
-  int32_t skipped_primitives;           /* analytic primitives not loaded */
+  int32_t primitives;                   /* analytic primitives loaded */
   int32_t missing_files;                /* .mat / .obj files that did not open */
   int32_t instances;                    /* instance lines (0: one identity instance per mesh) */
+  int32_t unknown_materials;            /* primitive lines naming no known material */
   float eye[3], target[3], up[3], fov;  /* the camera directive, for other aspect ratios */
 } xnh_file_info;
 int xnh_scene_load_file(const char* path, xnh_scene** out, xnh_camera* camera, xnh_file_info* info);

@@ -120,6 +120,18 @@ class Lib:
         return out
 
 
+def _prims_export(lib, h, n):
+    """(type, mat_id) pairs and 10 parameters per analytic primitive, as the
+    reference's Scene::all_prims holds them (reference build only)."""
+    tm = np.zeros((n, 2), np.int32)
+    p = np.zeros((n, 10), np.float32)
+    if n:
+        fn = lib.dll.orc_prims_export
+        fn.restype, fn.argtypes = None, [_vp, _ip, _fp]
+        fn(h, tm.ctypes.data_as(_ip), p.ctypes.data_as(_fp))
+    return tm, p
+
+
 class OScene:
     """A scene built inside one checker library from a SceneDesc."""
 
@@ -151,6 +163,16 @@ class OScene:
         z, h_, g = (np.array(v, np.float32) for v in (sk.zenith, sk.horizon, sk.ground))
         d.orc_scene_set_sky(self.h, z.ctypes.data_as(_fp), h_.ctypes.data_as(_fp), g.ctypes.data_as(_fp),
                             sk.intensity, sk.sharpness)
+        prims = getattr(desc, "prims", [])
+        if prims:
+            if not lib.is_reference:
+                raise NotImplementedError("analytic primitives are checked against the reference build only")
+            fn = d.orc_scene_add_prim
+            fn.restype, fn.argtypes = C.c_int, [_vp, C.c_int, _fp, C.c_int]
+            for pr in prims:
+                q = np.zeros(10, np.float32)
+                q[:len(pr.params)] = pr.params
+                fn(self.h, pr.type_id, q.ctypes.data_as(_fp), pr.mat)
         for ins in desc.instances:
             t = np.array(ins.translation, np.float32)
             r = np.array(ins.rot_deg, np.float32)
@@ -194,6 +216,9 @@ class OScene:
         self.lib.dll.orc_instance_export(self.h, xf.ctypes.data_as(_fp), ids.ctypes.data_as(_ip),
                                          meshes.ctypes.data_as(_ip))
         return nodes, order, xf, ids, meshes
+
+    def prims(self):
+        return _prims_export(self.lib, self.h, len(getattr(self.desc, "prims", [])))
 
     def normals(self, mesh: int) -> np.ndarray:
         out = np.zeros((self.desc.meshes[mesh].positions.shape[0], 3), np.float32)
@@ -338,6 +363,9 @@ class RefFileScene:
         out = np.zeros(11, np.float32)
         self.lib.dll.orc_ref_sky_export(self.h, out.ctypes.data_as(_fp))
         return out
+
+    def prims(self):
+        return _prims_export(self.lib, self.h, self.num_prims)
 
     def camera(self) -> np.ndarray:
         out = np.zeros(12, np.float32)

@@ -290,6 +290,40 @@ void orc_bsdf_eval(const float* p, int n, const float* wo, const float* wi, floa
   }
 }
 
+// ---- analytic primitives (reference only; not part of oracle_api.h) ------
+// One primitive into Scene::all_prims (geometry/prim_blas.h:54-61); p as
+// xnh_prim (sphere c r | plane n offset | box c half | disk c n r | quad o u v).
+int orc_scene_add_prim(void* h, int type, const float* p, int mat) {
+  xn::PrimGroup& g = R(h)->scene.all_prims;
+  switch (type) {
+    case 0: { xn::Sphere s; s.center = xn::Vec3(p[0], p[1], p[2]); s.radius = p[3]; s.mat_id = mat; g.add(s); break; }
+    case 1: { xn::Plane q; q.normal = xn::Vec3(p[0], p[1], p[2]); q.offset = p[3]; q.mat_id = mat; g.add(q); break; }
+    case 2: { xn::Box b; b.center = xn::Vec3(p[0], p[1], p[2]); b.half = xn::Vec3(p[3], p[4], p[5]); b.mat_id = mat; g.add(b); break; }
+    case 3: { xn::Disk d; d.center = xn::Vec3(p[0], p[1], p[2]); d.normal = xn::Vec3(p[3], p[4], p[5]); d.radius = p[6]; d.mat_id = mat; g.add(d); break; }
+    case 4: { xn::Quad q; q.origin = xn::Vec3(p[0], p[1], p[2]); q.u_vec = xn::Vec3(p[3], p[4], p[5]); q.v_vec = xn::Vec3(p[6], p[7], p[8]); q.mat_id = mat; g.add(q); break; }
+    default: return -1;
+  }
+  return g.size() - 1;
+}
+
+// Scene::all_prims back out: per primitive its type (PrimVariant index, the
+// xnh_prim type numbering), mat_id and parameters in orc_scene_add_prim order.
+void orc_prims_export(void* h, int* type_mat, float* p10) {
+  const auto& P = R(h)->scene.all_prims.prims;
+  for (size_t i = 0; i < P.size(); ++i) {
+    float* o = p10 + 10 * i;
+    for (int k = 0; k < 10; ++k) o[k] = 0.f;
+    auto put = [&](int at, const xn::Vec3& v) { o[at] = v.x; o[at + 1] = v.y; o[at + 2] = v.z; };
+    type_mat[2 * i] = static_cast<int>(P[i].index());
+    type_mat[2 * i + 1] = xn::prim_mat_id(P[i]);
+    if (auto* s = std::get_if<xn::Sphere>(&P[i])) { put(0, s->center); o[3] = s->radius; }
+    else if (auto* q = std::get_if<xn::Plane>(&P[i])) { put(0, q->normal); o[3] = q->offset; }
+    else if (auto* b = std::get_if<xn::Box>(&P[i])) { put(0, b->center); put(3, b->half); }
+    else if (auto* d = std::get_if<xn::Disk>(&P[i])) { put(0, d->center); put(3, d->normal); o[6] = d->radius; }
+    else if (auto* u = std::get_if<xn::Quad>(&P[i])) { put(0, u->origin); put(3, u->u_vec); put(6, u->v_vec); }
+  }
+}
+
 // ---- scene files (reference only; not part of oracle_api.h) --------------
 // load_scene (scene/scene_file.cpp:49-240) into a new handle; cfg5 receives
 // SceneConfig's width, height, samples, min_bounces, max_bounces. NULL when

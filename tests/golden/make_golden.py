@@ -2,9 +2,9 @@
 render core (oracle/_ref/libxenon_ref.so, compiled from the reference
 sources by oracle/Makefile). Run in a container that has the reference:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [name ...]
 
-Each fixture holds the inputs (a scene named by generator + arguments and
+(no names: every fixture). Each fixture holds the inputs (a scene named by generator + arguments and
 its array digest, a ray batch) and the reference's outputs (closest hits,
 occlusion, BLAS/TLAS digests, a small averaged image).
 """
@@ -20,7 +20,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-from raysets import bounce_rays, shadow_rays, standard_batch  # noqa: E402
+from raysets import bounce_rays, prim_rays, shadow_rays, standard_batch  # noqa: E402
 
 import pyoracle  # noqa: E402
 from xenon_amd import scenes  # noqa: E402
@@ -30,7 +30,10 @@ SCENES = {
     "big_mesh": ("big_mesh", dict(width=16, height=9, spp=4, level=4)),
     "instancing": ("instancing", dict(width=16, height=9, spp=4, n_inst=96, tri_range=(400, 2000), n_blas=6,
                                       n_emit=12)),
+    "prim_shapes": ("prim_shapes", dict(width=48, height=27, spp=16, n_spheres=24)),
 }
+# camera rays per fixture (as many bounce rays follow)
+RAYS = {"prim_shapes": 2048}
 
 
 def make_scene(name):
@@ -49,6 +52,13 @@ def scene_digest(desc) -> str:
         h.update(np.array([lt.mesh, lt.tri, *lt.emission, lt.area], np.float64).tobytes())
     for ins in desc.instances:
         h.update(np.array([ins.mesh, *ins.translation, ins.scale, *ins.rot_deg, ins.instance_id], np.float64).tobytes())
+    if desc.prims:   # scenes with primitives also digest them, the suns and the sky
+        for pr in desc.prims:
+            h.update(np.array([pr.type_id, pr.mat, *pr.params], np.float64).tobytes())
+        for su in desc.suns:
+            h.update(np.array([*su.direction, *su.color, su.intensity], np.float64).tobytes())
+        sk = desc.sky
+        h.update(np.array([*sk.zenith, *sk.horizon, *sk.ground, sk.intensity, sk.sharpness], np.float64).tobytes())
     return h.hexdigest()
 
 
@@ -63,14 +73,17 @@ def tree_digest(osc, desc) -> str:
     return h.hexdigest()
 
 
-def main():
+def main(names):
     ref = pyoracle.Lib("ref")
-    for name in SCENES:
+    for name in names or SCENES:
         desc = make_scene(name)
         osc = ref.load(desc)
-        rays = standard_batch(desc, osc.camera(), 768, seed=11)
+        n = RAYS.get(name, 768)
+        rays = standard_batch(desc, osc.camera(), n, seed=11)
+        if desc.prims:
+            rays = np.concatenate([rays, prim_rays(desc, n, np.random.default_rng(14))])
         hf, hi = osc.trace_closest(rays)
-        rays2 = bounce_rays(hf, hi, 768, np.random.default_rng(12))
+        rays2 = bounce_rays(hf, hi, n, np.random.default_rng(12))
         rays = np.concatenate([rays, rays2])
         hf, hi = osc.trace_closest(rays)
         srays = shadow_rays(rays, hf, hi, np.random.default_rng(13))
@@ -84,4 +97,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

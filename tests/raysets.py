@@ -25,6 +25,18 @@ def scene_bounds(desc):
         for m in desc.meshes:
             lo = np.minimum(lo, m.positions.min(0))
             hi = np.maximum(hi, m.positions.max(0))
+    for pr in getattr(desc, "prims", []):   # planes are unbounded: left out
+        p = np.asarray(pr.params, np.float64)
+        if pr.kind in ("sphere", "disk"):
+            pts = np.stack([p[:3] - p[-1], p[:3] + p[-1]])
+        elif pr.kind == "box":
+            pts = np.stack([p[:3] - p[3:6], p[:3] + p[3:6]])
+        elif pr.kind == "quad":
+            pts = np.stack([p[:3], p[:3] + p[3:6], p[:3] + p[6:9], p[:3] + p[3:6] + p[6:9]])
+        else:
+            continue
+        lo = np.minimum(lo, pts.min(0))
+        hi = np.maximum(hi, pts.max(0))
     return lo, hi
 
 
@@ -110,6 +122,53 @@ def shadow_rays(rays, hf, hi, rng):
     dist[dist == 0] = 1
     d = (d / dist[:, None]).astype(F32)
     return np.concatenate([o, d, np.full((n, 1), 5e-4, F32), (dist - F32(5e-4))[:, None]], 1).astype(F32)
+
+
+def prim_rays(desc, n, rng):
+    """Rays from random points of the scene box aimed at points of the
+    finite analytic primitives; half aim at a sphere's surface, a box's
+    edges and corners, a disk's rim or a quad's edges, where the slab,
+    radius and edge tests decide on equality."""
+    lo, hi = scene_bounds(desc)
+    fin = [p for p in desc.prims if p.kind != "plane"]
+    out = np.zeros((n, 8), F32)
+    for k in range(n):
+        pr = fin[rng.integers(0, len(fin))]
+        p = np.asarray(pr.params, np.float64)
+        edge = rng.random() < 0.5
+        if pr.kind == "sphere":
+            v = rng.normal(size=3)
+            tgt = p[:3] + p[3] * v / np.linalg.norm(v)
+        elif pr.kind == "box":
+            s = rng.uniform(-1.0, 1.0, 3)
+            ax = rng.permutation(3)
+            s[ax[0]] = rng.choice([-1.0, 1.0])
+            if edge:
+                s[ax[1]] = rng.choice([-1.0, 1.0])
+                if rng.random() < 0.3:
+                    s[ax[2]] = rng.choice([-1.0, 1.0])
+            tgt = p[:3] + s * p[3:6]
+        elif pr.kind == "disk":
+            nn = p[3:6] / np.linalg.norm(p[3:6])
+            t1 = np.cross(nn, [1.0, 0.0, 0.0] if abs(nn[0]) < 0.9 else [0.0, 1.0, 0.0])
+            t1 /= np.linalg.norm(t1)
+            t2 = np.cross(nn, t1)
+            a = rng.uniform(0.0, 2.0 * np.pi)
+            rad = p[6] if edge else p[6] * np.sqrt(rng.random())
+            tgt = p[:3] + rad * (np.cos(a) * t1 + np.sin(a) * t2)
+        else:   # quad
+            a, b = rng.random(2)
+            if edge:
+                a = float(rng.choice([0.0, 1.0]))
+                if rng.random() < 0.5:
+                    a, b = b, a
+            tgt = p[:3] + a * p[3:6] + b * p[6:9]
+        org = lo + (rng.random(3) * 1.2 - 0.1) * (hi - lo)
+        d = tgt - org
+        out[k, :3] = org
+        out[k, 3:6] = d / np.linalg.norm(d)
+        out[k, 6:8] = (1e-4, np.inf)
+    return out
 
 
 def standard_batch(desc, cam12, n_each=4096, seed=0):

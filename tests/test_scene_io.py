@@ -65,10 +65,12 @@ def assert_same_scene(hs, rs):
             assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
         (n1, s1), (n2, s2) = hs.blas(i), rs.blas(i)
         assert n1.tobytes() == n2.tobytes() and np.array_equal(s1, s2)
-    assert info.skipped_primitives == rs.num_prims
-    if rs.num_prims == 0:   # the reference adds one instance over its analytic primitives
-        for a, b in zip(hs.tlas(), rs.tlas()):
-            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+    assert info.primitives == f.num_prims == rs.num_prims
+    for a, b in zip(hs.prims(), rs.prims()):
+        assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+    # the TLAS, the primitives' instance (mesh -1) included
+    for a, b in zip(hs.tlas(), rs.tlas()):
+        assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
     L = rs.lights()
     assert f.num_lights == len(L)
     for i in range(f.num_lights):
@@ -227,7 +229,7 @@ def test_written_scene_every_directive(reference, tmp_path, monkeypatch):
     assert hs.info.missing_files == 3          # missing.mat, blob.obj (mesh paths are not
     # resolved beside the scene file), meshes/absent.obj
     assert hs.info.meshes == 4
-    assert hs.info.skipped_primitives == 2
+    assert hs.info.primitives == 2
     assert_same_scene(hs, rs)
 
 

@@ -27,6 +27,8 @@
 //   lights  4 x float4 per light: {p0, area} {p1, e.r} {p2, e.g} {n_geo, e.b}
 //   mats    4 x float4 per material (see DevMat)
 //   suns    2 x float4 per directional light: {dir, 0} {emission, 0}
+//   prims   analytic primitives with their PrimBLAS (pnodes, porder); the
+//           instance of the primitives has root kPrimRoot
 #pragma once
 #include <cstdint>
 
@@ -41,6 +43,10 @@ constexpr uint32_t kEntShift = 28;
 constexpr uint32_t kEntIndexMask = (1u << kEntShift) - 1u;
 constexpr uint32_t kEntBigLeaf = 15;
 constexpr uint32_t kNoRoot = 0xFFFFFFFFu;
+// Instance root of the scene's analytic primitives (its PrimBLAS), and the
+// flag that marks a primitive in a hit's slot (slot = primitive index).
+constexpr uint32_t kPrimRoot = 0xFFFFFFFEu;
+constexpr uint32_t kPrimHit = 0x80000000u;
 // Bound on the walk rounds of one wave in a launch: far above any
 // legitimate walk (a wave's share of a batch takes ~1e5 rounds), it makes
 // every wave exit even if a round stopped making progress; a wave that
@@ -66,6 +72,10 @@ struct DevScene {
   const float4* lights;
   const float4* mats;
   const float4* suns;
+  const float4* prims;      // 3 x float4 per primitive: {type, mat, p0, p1} {p2..p5} {p6..p9}
+  const float4* pnodes;     // 2 x float4 per PrimBLAS node, as bnodes without the pair layout
+  const uint32_t* porder;   // PrimBLAS leaf slot -> primitive index
+  int32_t num_pnodes;
   int32_t num_tnodes;
   int32_t num_lights;
   int32_t num_suns;

@@ -174,6 +174,195 @@ __device__ __forceinline__ InstRec load_inst(const float4* inst, int i) {
   return r;
 }
 
+// ---- analytic primitives ------------------------------------------------------
+// geometry/primitives.h:54-318 and geometry/prim_blas.cpp:165-229, with the
+// reference's operation order (true divisions, std::min/std::max and the
+// first-largest rule of std::max({...})).
+struct PrimRec {
+  int type, mat;
+  float q[10];
+};
+__device__ __forceinline__ PrimRec load_prim(const float4* P, uint32_t i) {
+  const float4 a = gld4(P, 3 * static_cast<size_t>(i)), b = gld4(P, 3 * static_cast<size_t>(i) + 1),
+               c = gld4(P, 3 * static_cast<size_t>(i) + 2);
+  PrimRec r;
+  r.type = __float_as_int(a.x);
+  r.mat = __float_as_int(a.y);
+  r.q[0] = a.z; r.q[1] = a.w; r.q[2] = b.x; r.q[3] = b.y; r.q[4] = b.z;
+  r.q[5] = b.w; r.q[6] = c.x; r.q[7] = c.y; r.q[8] = c.z; r.q[9] = c.w;
+  return r;
+}
+
+// AABB::intersect (geometry/aabb.h:34-47) on a PrimBLAS node.
+__device__ __forceinline__ bool prim_node_hit(float4 na, float4 nb, f3 o, f3 d, float rtmin, float rtmax,
+                                              float& tnear) {
+  const float tx0 = (na.x - o.x) / d.x, tx1 = (na.w - o.x) / d.x;
+  const float ty0 = (na.y - o.y) / d.y, ty1 = (nb.x - o.y) / d.y;
+  const float tz0 = (na.z - o.z) / d.z, tz1 = (nb.y - o.z) / d.z;
+  float lo = smin(tx0, tx1), hi = smax(tx0, tx1);
+  const float ly = smin(ty0, ty1), lz = smin(tz0, tz1), hy = smax(ty0, ty1), hz = smax(tz0, tz1);
+  if (lo < ly) lo = ly;
+  if (lo < lz) lo = lz;
+  if (lo < rtmin) lo = rtmin;
+  if (hy < hi) hi = hy;
+  if (hz < hi) hi = hz;
+  if (rtmax < hi) hi = rtmax;
+  tnear = lo;
+  return lo <= hi;
+}
+
+// Box slab walk (primitives.h:159-182): the entry distance and the entering
+// face (axis, and whether it is the low face).
+__device__ __forceinline__ bool prim_box_slabs(const PrimRec& p, f3 o, f3 d, float tmin, float rect, float& t,
+                                               int& axis, bool& low) {
+  const f3 c{p.q[0], p.q[1], p.q[2]}, h{p.q[3], p.q[4], p.q[5]};
+  const f3 mn = c - h, mx = c + h;
+  float t_lo = tmin, t_hi = rect;
+  axis = 0;
+  low = false;
+  for (int i = 0; i < 3; ++i) {
+    const float inv = 1.f / comp(d, i);
+    float t0 = (comp(mn, i) - comp(o, i)) * inv;
+    float t1 = (comp(mx, i) - comp(o, i)) * inv;
+    const bool flip = inv < 0.f;
+    if (flip) {
+      const float tmp = t0;
+      t0 = t1;
+      t1 = tmp;
+    }
+    if (t0 > t_lo) {
+      t_lo = t0;
+      axis = i;
+      low = !flip;
+    }
+    t_hi = smin(t_hi, t1);
+    if (t_lo > t_hi) return false;
+  }
+  t = t_lo;
+  return true;
+}
+
+// The primitive's intersect: a hit at t in [tmin, rect).
+__device__ __forceinline__ bool prim_hit(const PrimRec& p, f3 o, f3 d, float tmin, float rect, float& t) {
+  switch (p.type) {
+    case 0: {   // sphere
+      const f3 c{p.q[0], p.q[1], p.q[2]};
+      const float r = p.q[3];
+      const f3 oc = o - c;
+      const float a = dot(d, d), hb = dot(oc, d), cc = dot(oc, oc) - r * r;
+      const float disc = hb * hb - a * cc;
+      if (disc < 0.f) return false;
+      const float sq = sqrtf(disc);
+      t = (-hb - sq) / a;
+      if (t < tmin || t >= rect) {
+        t = (-hb + sq) / a;
+        if (t < tmin || t >= rect) return false;
+      }
+      return true;
+    }
+    case 1: {   // plane
+      const f3 n{p.q[0], p.q[1], p.q[2]};
+      const float denom = dot(n, d);
+      if (fabsf(denom) < 1e-7f) return false;
+      t = (p.q[3] - dot(n, o)) / denom;
+      return !(t < tmin || t >= rect);
+    }
+    case 2: {   // box
+      int axis;
+      bool low;
+      return prim_box_slabs(p, o, d, tmin, rect, t, axis, low);
+    }
+    case 3: {   // disk
+      const f3 c{p.q[0], p.q[1], p.q[2]}, n{p.q[3], p.q[4], p.q[5]};
+      const float denom = dot(n, d);
+      if (fabsf(denom) < 1e-7f) return false;
+      t = dot(c - o, n) / denom;
+      if (t < tmin || t >= rect) return false;
+      const f3 op = (o + t * d) - c;
+      return !(dot(op, op) > p.q[6] * p.q[6]);
+    }
+    default: {   // quad
+      const f3 org{p.q[0], p.q[1], p.q[2]}, u{p.q[3], p.q[4], p.q[5]}, v{p.q[6], p.q[7], p.q[8]};
+      const f3 nr = cross(u, v);
+      const float area_sq = dot(nr, nr);
+      if (area_sq < 1e-14f) return false;
+      const float denom = dot(nr, d);
+      if (fabsf(denom) < 1e-9f) return false;
+      t = dot(nr, org - o) / denom;
+      if (t < tmin || t >= rect) return false;
+      const f3 op = (o + t * d) - org;
+      const float s = dot(cross(op, v), nr) / area_sq;
+      const float tv = dot(cross(u, op), nr) / area_sq;
+      return !(s < 0.f || s > 1.f || tv < 0.f || tv > 1.f);
+    }
+  }
+}
+
+// Normal (facing the ray) and (u, v) the primitive's intersect writes for a
+// hit at t.
+__device__ __forceinline__ f3 prim_normal(const PrimRec& p, f3 o, f3 d, float tmin, float t, float& u, float& v) {
+  constexpr float kPi = 3.14159265358979323846f;
+  constexpr float kInvPi = 1.f / kPi;
+  const f3 pos = o + t * d;
+  switch (p.type) {
+    case 0: {
+      const f3 c{p.q[0], p.q[1], p.q[2]};
+      const f3 out = (pos - c) / p.q[3];
+      const float phi = atan2f(-out.z, out.x) + kPi;
+      const float theta = acosf(fminf(fmaxf(-out.y, -1.f), 1.f));
+      u = phi * (0.5f * kInvPi);
+      v = theta * kInvPi;
+      return dot(out, d) < 0.f ? out : -out;
+    }
+    case 1: {
+      const f3 n{p.q[0], p.q[1], p.q[2]};
+      u = pos.x - floorf(pos.x);
+      v = pos.z - floorf(pos.z);
+      return dot(n, d) < 0.f ? n : -n;
+    }
+    case 2: {
+      float tt;
+      int axis;
+      bool low;
+      prim_box_slabs(p, o, d, tmin, __builtin_huge_valf(), tt, axis, low);
+      f3 n{0.f, 0.f, 0.f};
+      const float s = low ? -1.f : 1.f;
+      if (axis == 0) n.x = s;
+      else if (axis == 1) n.y = s;
+      else n.z = s;
+      if (dot(n, d) > 0.f) n = -n;
+      const f3 c{p.q[0], p.q[1], p.q[2]}, h{p.q[3], p.q[4], p.q[5]};
+      const f3 mn = c - h;
+      const int ua = (axis + 1) % 3, va = (axis + 2) % 3;
+      u = (comp(pos, ua) - comp(mn, ua)) / (2.f * comp(h, ua));
+      v = (comp(pos, va) - comp(mn, va)) / (2.f * comp(h, va));
+      return n;
+    }
+    case 3: {
+      const f3 c{p.q[0], p.q[1], p.q[2]}, n{p.q[3], p.q[4], p.q[5]};
+      const f3 op = pos - c;
+      const f3 ref = fabsf(n.x) < 0.9f ? f3{1.f, 0.f, 0.f} : f3{0.f, 1.f, 0.f};
+      const f3 tang = normalize(cross(n, ref));
+      const f3 bita = cross(n, tang);
+      float phi = atan2f(dot(op, bita), dot(op, tang));
+      if (phi < 0.f) phi += 2.f * kPi;
+      u = phi * (0.5f * kInvPi);
+      v = sqrtf(dot(op, op)) / p.q[6];
+      return dot(n, d) < 0.f ? n : -n;
+    }
+    default: {
+      const f3 org{p.q[0], p.q[1], p.q[2]}, uu{p.q[3], p.q[4], p.q[5]}, vv{p.q[6], p.q[7], p.q[8]};
+      const f3 nr = cross(uu, vv);
+      const float area_sq = dot(nr, nr);
+      const f3 op = pos - org;
+      u = dot(cross(op, vv), nr) / area_sq;
+      v = dot(cross(uu, op), nr) / area_sq;
+      const f3 n = normalize(nr);
+      return dot(nr, d) > 0.f ? -n : n;
+    }
+  }
+}
+
 // Scene walk as a resumable state machine, advanced one round at a time so
 // a persistent kernel can retire a finished ray and start the next in the
 // same lane while the rest of the wave keeps walking.
@@ -207,11 +396,14 @@ constexpr uint32_t kEntEnd = kNoRoot;   // no entry left in the BLAS walk
 __device__ __forceinline__ bool ent_interior(uint32_t e) { return e != kEntEnd && (e >> kEntShift) == 0u; }
 __device__ __forceinline__ bool ent_leaf(uint32_t e) { return e != kEntEnd && (e >> kEntShift) != 0u; }
 
-template <bool kAny, bool kCount = true>
+// kPrims: the scene may hold analytic primitives (a variant, because their
+// walk raises the register count of the whole kernel).
+template <bool kAny, bool kCount = true, bool kPrims = true>
 struct Walker {
   f3 o, d, inv;       // world ray
   f3 lo, ld, linv;    // instance-space ray of the BLAS walk in progress
   float tmin, tbest;
+  float rtmax;        // the ray's tmax (bounds the primitives' box tests, aabb.h:43)
   int blas_base;      // stack depth under the current BLAS walk; -1 in the TLAS
   int inst;           // instance of the BLAS walk
   uint32_t e;         // next BLAS entry to process, or kEntEnd; a leaf only while `leaf` is held
@@ -226,6 +418,7 @@ struct Walker {
     d = d_;
     inv = f3{1.f / d.x, 1.f / d.y, 1.f / d.z};
     tmin = tmin_;
+    rtmax = tmax_;
     tbest = kAny ? tmax_ : __builtin_huge_valf();
     blas_base = -1;
     inst = -1;
@@ -250,6 +443,72 @@ struct Walker {
 
   __device__ __forceinline__ uint32_t pop_blas(TravStack& st) { return st.sp > blas_base ? st.pop() : kEntEnd; }
 
+  // The scene's primitives, walked to the end at once in the instance-space
+  // ray lo/ld (PrimBLAS::intersect / ::intersects, prim_blas.cpp:165-229):
+  // nodes tested when popped, children tested (closest hit) and pushed
+  // nearer last; true when an any-hit walk found an occluder.
+  __device__ __forceinline__ bool prim_walk(const DevScene& S, TravStack& st, TravCount& c) {
+    if (S.num_pnodes <= 0) return false;
+    const int base = st.sp;
+    st.push(0u);
+    while (st.sp > base && !st.overflow) {
+      const uint32_t idx = st.pop();
+      const float4 na = gld4(S.pnodes, 2 * static_cast<size_t>(idx)), nb = gld4(S.pnodes, 2 * static_cast<size_t>(idx) + 1);
+      float tn;
+      if (kCount) ++c.node_tests;
+      if (!prim_node_hit(na, nb, lo, ld, tmin, rtmax, tn)) continue;
+      if (!kAny && tn > tbest) continue;
+      const uint32_t child = __float_as_uint(nb.z), count = __float_as_uint(nb.w);
+      if (count > 0) {
+        for (uint32_t k = 0; k < count; ++k) {
+          const uint32_t pi = gld1(S.porder, child + k);
+          const PrimRec pr = load_prim(S.prims, pi);
+          float t;
+          if (kCount) ++c.tri_tests;
+          if (prim_hit(pr, lo, ld, tmin, kAny ? rtmax : tbest, t)) {
+            if (kAny) return true;   // the ray is finished; its stack is reset by the next begin()
+            tbest = t;
+            hit.u = 0.f;
+            hit.v = 0.f;
+            hit.slot = pi | kPrimHit;
+            inst_found = true;
+          }
+        }
+        continue;
+      }
+      if (kAny) {
+        st.push(child);
+        st.push(child + 1);
+        continue;
+      }
+      if (kCount) {
+        ++c.interior;
+        c.node_tests += 2;
+      }
+      float tl, tr;
+      const bool hl = prim_node_hit(gld4(S.pnodes, 2 * static_cast<size_t>(child)),
+                                    gld4(S.pnodes, 2 * static_cast<size_t>(child) + 1), lo, ld, tmin, rtmax, tl) &&
+                      tl <= tbest;
+      const bool hr = prim_node_hit(gld4(S.pnodes, 2 * static_cast<size_t>(child) + 2),
+                                    gld4(S.pnodes, 2 * static_cast<size_t>(child) + 3), lo, ld, tmin, rtmax, tr) &&
+                      tr <= tbest;
+      if (hl && hr) {
+        if (tl < tr) {
+          st.push(child + 1);
+          st.push(child);
+        } else {
+          st.push(child);
+          st.push(child + 1);
+        }
+      } else if (hl) {
+        st.push(child);
+      } else if (hr) {
+        st.push(child + 1);
+      }
+    }
+    return false;
+  }
+
   // TLAS steps until this lane is inside a BLAS walk; true = ray finished.
   __device__ __forceinline__ bool tlas_steps(const DevScene& S, TravStack& st, TravCount& c) {
     while (blas_base < 0) {
@@ -265,6 +524,20 @@ struct Walker {
         fast = all_finite(linv) && all_finite(ld) && all_finite(lo);
         inst = ii;
         inst_found = false;
+        if (in.root == kPrimRoot) {   // the primitives: walked here, to the end
+          if (kPrims) {
+            if (prim_walk(S, st, c)) {
+              found = true;
+              return true;
+            }
+            if (!kAny && inst_found) {
+              hit.inst = ii;
+              found = true;
+            }
+            if (st.overflow) return true;
+          }
+          continue;
+        }
         blas_base = st.sp;
         // e never holds a leaf unless one is already held: a root that is a
         // leaf is held at once (kNoRoot, an empty mesh, is kEntEnd)

@@ -82,8 +82,33 @@ __device__ __forceinline__ f3 normal_out(const InstRec& in, f3 n) {
                       in.c2.x * n.x + in.c2.y * n.y + in.c2.z * n.z});
 }
 
-__device__ __forceinline__ Surface resolve(const DevScene& S, const HitRef& h, f3 o, f3 d) {
+// A primitive hit: the primitive's own normal (facing the ray) and uv in
+// instance space, then TLAS::fixup_hit (tlas.cpp:156-168).
+__device__ __forceinline__ Surface resolve_prim(const DevScene& S, const HitRef& h, const InstRec& in, f3 o, f3 d,
+                                               float tmin, float& u, float& v) {
+  const uint32_t pi = h.slot & ~kPrimHit;
+  const PrimRec pr = load_prim(S.prims, pi);
+  const f3 lo = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
+  const f3 ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
+  const f3 nl = prim_normal(pr, lo, ld, tmin, h.t, u, v);
+  Surface s;
+  s.tri = pi;
+  s.pos = o + h.t * d;
+  s.n = normal_out(in, nl);
+  s.gn = s.n;
+  s.front = dot(s.gn, d) < 0.f;
+  s.mat = pr.mat;
+  s.inst_id = in.id;
+  s.lmap = -1;
+  return s;
+}
+
+__device__ __forceinline__ Surface resolve(const DevScene& S, const HitRef& h, f3 o, f3 d, float tmin = 0.f) {
   const InstRec in = load_inst(S.inst, h.inst);
+  if (h.slot & kPrimHit) {
+    float u, v;
+    return resolve_prim(S, h, in, o, d, tmin, u, v);
+  }
   const float4 a = S.tris[3 * h.slot], b = S.tris[3 * h.slot + 1], c = S.tris[3 * h.slot + 2];
   const f3 e1{a.w, b.x, b.y}, e2{b.z, b.w, c.x};
   Surface s;
@@ -115,8 +140,13 @@ struct GeoSurface {
   int32_t lmap;
 };
 
-__device__ __forceinline__ GeoSurface resolve_geo(const DevScene& S, const HitRef& h, f3 o, f3 d) {
+__device__ __forceinline__ GeoSurface resolve_geo(const DevScene& S, const HitRef& h, f3 o, f3 d, float tmin = 0.f) {
   const InstRec in = load_inst(S.inst, h.inst);
+  if (h.slot & kPrimHit) {
+    float u, v;
+    const Surface f = resolve_prim(S, h, in, o, d, tmin, u, v);
+    return GeoSurface{f.pos, f.gn, f.mat, f.tri, -1};
+  }
   const float4 a = S.tris[3 * h.slot], b = S.tris[3 * h.slot + 1], c = S.tris[3 * h.slot + 2];
   const f3 e1{a.w, b.x, b.y}, e2{b.z, b.w, c.x};
   GeoSurface s;

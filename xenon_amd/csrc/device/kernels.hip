@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
 // seg: queue entries per chunk sample slot (1 path queue, 2 shadow queue).
 // Must be reached by every lane of the wave.
 // ---------------------------------------------------------------------------
-template <bool kAny, bool kCount, class Load, class Finish>
+template <bool kAny, bool kCount, bool kPrims, class Load, class Finish>
 __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchView& v, uint32_t* cursor,
                                             uint32_t seg, TravStack& st, TravCount& c, uint32_t& rays,
                                             bool& overflow, Load load, Finish finish) {
@@ -106,7 +106,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
   const int lane = lane_id();
   const unsigned long long below = (1ull << lane) - 1ull;
   uint32_t my = kIdle;
-  Walker<kAny, kCount> w;
+  Walker<kAny, kCount, kPrims> w;
   uint32_t pnext = 0, pend = 0;   // wave-uniform pool of offsets in chunk pci
   int pci = 0;
   int held = -1;                  // fetched unit of another job, waiting for the wave to drain
@@ -173,7 +173,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
 // closest: persistent closest-hit traversal of queue qi (kCount: with the
 // traversal work counters).
 // ---------------------------------------------------------------------------
-template <bool kCount>
+template <bool kCount, bool kPrims>
 __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int cursor, int lds_depth) {
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
@@ -187,7 +187,7 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
-  trace_queue<false, kCount>(
+  trace_queue<false, kCount, kPrims>(
       B, v, &B.counters[kCursor + cursor], 1u, st, c, rays, overflow,
       [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
         const float4 r0 = q.ray0[i], r1 = q.ray1[i];
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
         tmin = kBounceOffset;
         tmax = __builtin_huge_valf();
       },
-      [&](uint32_t i, const Walker<false, kCount>& w) {
+      [&](uint32_t i, const Walker<false, kCount, kPrims>& w) {
         B.hit[i] = make_float4(w.found ? w.tbest : 0.f, w.hit.u, w.hit.v, __int_as_float(w.found ? w.hit.inst : -1));
         B.hit_slot[i] = w.hit.slot;
       });
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
           B.l_term[slot] = make_float4(L.x, L.y, L.z, 0.f);
         }
       } else {
-        const Surface sf = resolve(S, h, o, d);
+        const Surface sf = resolve(S, h, o, d, kBounceOffset);
         int li = -1;
         if (sf.lmap >= 0) li = S.lmap[sf.lmap + static_cast<int>(sf.tri)];
         if (li >= 0) {
@@ -470,7 +470,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
 // shadow: persistent any-hit traversal; unoccluded contributions are added
 // to their path's per-slot NEE sum (one writer per (slot, light kind)).
 // ---------------------------------------------------------------------------
-template <bool kCount>
+template <bool kCount, bool kPrims>
 __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int lds_depth) {
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + kShCnt;
@@ -479,7 +479,7 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
-  trace_queue<true, kCount>(
+  trace_queue<true, kCount, kPrims>(
       B, v, &B.counters[kCursor + cursor], 2u, st, c, rays, overflow,
       [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
         const float4 r0 = B.sh0[i], r1 = B.sh1[i];
@@ -488,7 +488,7 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
         tmin = kShadowOff;
         tmax = r1.z;
       },
-      [&](uint32_t i, const Walker<true, kCount>& w) {
+      [&](uint32_t i, const Walker<true, kCount, kPrims>& w) {
         if (!w.found) {
           const uint32_t k = __float_as_uint(B.sh1[i].w);
           const float4 cv = B.sh_c[i];
@@ -584,10 +584,13 @@ __global__ void __launch_bounds__(256) k_api_closest(DevScene S, const float4* r
       for (int k = 6; k < 15; ++k) out[k] = 0.f;
       reinterpret_cast<int*>(out)[15] = 0;
     } else {
-      const Surface sf = resolve(S, h, o, d);
+      float u = h.u, v = h.v;
+      Surface sf;
+      if (h.slot & kPrimHit) sf = resolve_prim(S, h, load_inst(S.inst, h.inst), o, d, a.w, u, v);
+      else sf = resolve(S, h, o, d, a.w);
       out[0] = h.t;
-      out[1] = h.u;
-      out[2] = h.v;
+      out[1] = u;
+      out[2] = v;
       reinterpret_cast<int*>(out)[3] = sf.inst_id;
       reinterpret_cast<int*>(out)[4] = static_cast<int>(sf.tri);
       reinterpret_cast<int*>(out)[5] = sf.mat;
@@ -631,7 +634,7 @@ int trace_grid(int block, int lds_depth) {
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
   int per_cu = 0;
   const size_t lds = static_cast<size_t>(block) * lds_depth * sizeof(uint32_t);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_closest<false>), block, lds) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_closest<false, false>), block, lds) !=
       hipSuccess)
     return -1;
   if (per_cu < 1) per_cu = 1;
@@ -653,10 +656,15 @@ hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths) {
 }
 
 hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block,
-                          int lds_depth, bool count) {
+                          int lds_depth, bool count, bool prims) {
   const size_t lds = static_cast<size_t>(block) * lds_depth * 4;
-  if (count) hipLaunchKernelGGL(k_closest<true>, dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
-  else hipLaunchKernelGGL(k_closest<false>, dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+  if (prims) {
+    if (count) hipLaunchKernelGGL((k_closest<true, true>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+    else hipLaunchKernelGGL((k_closest<false, true>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+  } else {
+    if (count) hipLaunchKernelGGL((k_closest<true, false>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+    else hipLaunchKernelGGL((k_closest<false, false>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+  }
   return hipGetLastError();
 }
 
@@ -667,10 +675,15 @@ hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, 
 }
 
 hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth,
-                         bool count) {
+                         bool count, bool prims) {
   const size_t lds = static_cast<size_t>(block) * lds_depth * 4;
-  if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(grid), dim3(block), lds, s, B, cursor, lds_depth);
-  else hipLaunchKernelGGL(k_shadow<false>, dim3(grid), dim3(block), lds, s, B, cursor, lds_depth);
+  if (prims) {
+    if (count) hipLaunchKernelGGL((k_shadow<true, true>), dim3(grid), dim3(block), lds, s, B, cursor, lds_depth);
+    else hipLaunchKernelGGL((k_shadow<false, true>), dim3(grid), dim3(block), lds, s, B, cursor, lds_depth);
+  } else {
+    if (count) hipLaunchKernelGGL((k_shadow<true, false>), dim3(grid), dim3(block), lds, s, B, cursor, lds_depth);
+    else hipLaunchKernelGGL((k_shadow<false, false>), dim3(grid), dim3(block), lds, s, B, cursor, lds_depth);
+  }
   return hipGetLastError();
 }
 

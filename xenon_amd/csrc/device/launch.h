@@ -13,10 +13,10 @@ int num_cus();
 
 hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths);
 hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block,
-                          int lds_depth, bool count);
+                          int lds_depth, bool count, bool prims);
 hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, int grid);
 hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth,
-                         bool count);
+                         bool count, bool prims);
 hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int max_pixels);
 hipError_t launch_advance(hipStream_t s, uint32_t* counters);
 hipError_t launch_average(hipStream_t s, const float4* accum, float* out, int npix, float inv_spp);

@@ -81,6 +81,7 @@ struct xnh_ctx {
   std::vector<hipEvent_t> prof_events;
   bool profiling = false;         // per-kernel HIP-event timing
   bool counting = false;          // traversal work counters (the counting kernel variants)
+  bool prims = false;             // a scene of the current jobs holds analytic primitives
   bool use_graph = true;
   xnh_gpu_scene own;              // the scene of xnh_upload_scene
   bool has_scene = false;
@@ -202,8 +203,35 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
     ns += static_cast<size_t>(me.num_triangles);
     nv += static_cast<size_t>(me.num_vertices);
   }
-  if (ns >= (size_t(1) << kEntShift) || nn >= (size_t(1) << kEntShift))
+  if (ns >= (size_t(1) << kEntShift) || nn + 2 >= (size_t(1) << kEntShift))
     return fail(c, XNH_E_INVALID, "too many triangles or BLAS nodes (limit 2^28)");
+  // analytic primitives and their PrimBLAS
+  const int np = fs->prims ? fs->num_prims : 0, npn = fs->prim_nodes ? fs->num_prim_nodes : 0;
+  // (a primitive index below 2^31 never reaches the kPrimHit flag bit)
+  if (np < 0 || npn < 0 || (np > 0 && (npn < 1 || !fs->prim_order)))
+    return fail(c, XNH_E_INVALID, "bad primitive arrays");
+  std::vector<float4> prims(3 * static_cast<size_t>(np)), pnodes(2 * static_cast<size_t>(npn));
+  std::vector<uint32_t> porder(static_cast<size_t>(np));
+  for (int i = 0; i < np; ++i) {
+    const xnh_prim& pr = fs->prims[i];
+    if (pr.type < XNH_PRIM_SPHERE || pr.type > XNH_PRIM_QUAD) return fail(c, XNH_E_INVALID, "unknown primitive type");
+    if (pr.mat_id < 0 || pr.mat_id >= fs->num_materials) return fail(c, XNH_E_INVALID, "primitive material out of range");
+    const float* q = pr.p;
+    prims[3 * i] = f4(ibits(pr.type), ibits(pr.mat_id), q[0], q[1]);
+    prims[3 * i + 1] = f4(q[2], q[3], q[4], q[5]);
+    prims[3 * i + 2] = f4(q[6], q[7], q[8], q[9]);
+    porder[i] = fs->prim_order[i];
+    if (porder[i] >= static_cast<uint32_t>(np)) return fail(c, XNH_E_INVALID, "primitive index out of range");
+  }
+  for (int j = 0; j < npn; ++j) {
+    const xnh_bvh_node& n = fs->prim_nodes[j];
+    if (n.count > 0 && static_cast<uint64_t>(n.child) + n.count > static_cast<uint64_t>(np))
+      return fail(c, XNH_E_INVALID, "PrimBLAS leaf outside primitive range");
+    if (n.count == 0 && (n.child + 1 >= static_cast<uint32_t>(npn) || n.child <= static_cast<uint32_t>(j)))
+      return fail(c, XNH_E_INVALID, "PrimBLAS child outside node range");
+    pnodes[2 * j] = f4(n.lo[0], n.lo[1], n.lo[2], n.hi[0]);
+    pnodes[2 * j + 1] = f4(n.hi[1], n.hi[2], bits(n.child), bits(n.count));
+  }
   std::vector<float4> bnodes(2 * nn), tris(3 * ns), vnrm(nv);
   std::vector<int32_t> vidx(3 * ns), tmat(ns);
   std::vector<bool> flat(nm, false);
@@ -276,10 +304,18 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   std::vector<float4> inst(4 * static_cast<size_t>(fs->num_instances));
   for (int i = 0; i < fs->num_instances; ++i) {
     const xnh_instance& in = fs->instances[i];
-    if (in.mesh < 0 || in.mesh >= nm) return fail(c, XNH_E_INVALID, "instance mesh out of range");
     if (torder.size() > 0 && torder[i] >= static_cast<uint32_t>(fs->num_instances))
       return fail(c, XNH_E_INVALID, "TLAS instance index out of range");
     const float* w = in.world_to_local;
+    if (in.mesh == XNH_PRIM_GEOMETRY) {   // the primitives (no area lights on them)
+      if (npn < 1) return fail(c, XNH_E_INVALID, "primitive instance without primitives");
+      inst[4 * i] = f4(w[0], w[1], w[2], w[3]);
+      inst[4 * i + 1] = f4(w[4], w[5], w[6], w[7]);
+      inst[4 * i + 2] = f4(w[8], w[9], w[10], w[11]);
+      inst[4 * i + 3] = f4(bits(kPrimRoot), ibits(in.instance_id), bits(0u), ibits(-1));
+      continue;
+    }
+    if (in.mesh < 0 || in.mesh >= nm) return fail(c, XNH_E_INVALID, "instance mesh out of range");
     int32_t lm = -1;
     const int ntri = fs->meshes[in.mesh].num_triangles;
     for (int l = 0; l < fs->num_lights; ++l) {
@@ -342,8 +378,8 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   free_all(g->allocs);
   DevScene S{};
-  float4 *p_bn, *p_tr, *p_tn, *p_in, *p_vn, *p_li, *p_ma, *p_su;
-  uint32_t* p_to;
+  float4 *p_bn, *p_tr, *p_tn, *p_in, *p_vn, *p_li, *p_ma, *p_su, *p_pr = nullptr, *p_pn = nullptr;
+  uint32_t *p_to, *p_po = nullptr;
   int32_t *p_vi, *p_tm, *p_lm;
   int rc = dev_upload(c, g->allocs, &p_bn, bnodes);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_tr, tris);
@@ -357,6 +393,9 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_li, lights);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_ma, mats);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_su, suns);
+  if (rc == XNH_OK && np > 0) rc = dev_upload(c, g->allocs, &p_pr, prims);
+  if (rc == XNH_OK && np > 0) rc = dev_upload(c, g->allocs, &p_pn, pnodes);
+  if (rc == XNH_OK && np > 0) rc = dev_upload(c, g->allocs, &p_po, porder);
   if (rc != XNH_OK) {
     free_all(g->allocs);
     return rc;
@@ -373,6 +412,10 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   S.lights = p_li;
   S.mats = p_ma;
   S.suns = p_su;
+  S.prims = p_pr;
+  S.pnodes = p_pn;
+  S.porder = p_po;
+  S.num_pnodes = np > 0 ? npn : 0;
   S.num_tnodes = ntn;
   S.num_lights = fs->num_lights;
   S.num_suns = fs->num_dir_lights;
@@ -385,8 +428,9 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   S.sky.intensity = fs->sky.intensity;
   g->S = S;
   g->bytes = 0;
-  for (const auto* v : {&bnodes, &tris, &tnodes, &inst, &vnrm, &lights, &mats, &suns}) g->bytes += v->size() * 16;
-  g->bytes += (torder.size() + vidx.size() + tmat.size() + lmap.size()) * 4;
+  for (const auto* v : {&bnodes, &tris, &tnodes, &inst, &vnrm, &lights, &mats, &suns, &prims, &pnodes})
+    g->bytes += v->size() * 16;
+  g->bytes += (torder.size() + vidx.size() + tmat.size() + lmap.size() + porder.size()) * 4;
   return XNH_OK;
 }
 
@@ -467,7 +511,7 @@ int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
   for (int d = 0; d < depth; ++d) {
     const int qi = d & 1;
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
-    HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth, c->counting));
+    HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth, c->counting, c->prims));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     if (c->guide.on && d < c->B.guide_depth)
       HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, d, c->guide.net, c->guide.feat, c->grid_shade * 2));
@@ -475,7 +519,7 @@ int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
     HIP_OR_FAIL(c, launch_shade(s, c->B, qi, d, c->grid_shade));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
-    HIP_OR_FAIL(c, launch_shadow(s, c->B, 2 * d + 1, c->grid_trace, c->block, c->lds_depth, c->counting));
+    HIP_OR_FAIL(c, launch_shadow(s, c->B, 2 * d + 1, c->grid_trace, c->block, c->lds_depth, c->counting, c->prims));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
   }
   HIP_OR_FAIL(c, launch_accumulate(s, c->B, cap));
@@ -622,10 +666,12 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
   int rc = ensure_tables(c, std::max<size_t>(jobs.size(), 1), std::max<size_t>(nb, 1));
   if (rc != XNH_OK) return rc;
   std::vector<DevJob> dj(jobs.size());
+  bool prims = false;
   for (size_t j = 0; j < jobs.size(); ++j) {
     DevJob& d = dj[j];
     d = DevJob{};
     d.S = *jobs[j].S;
+    prims = prims || d.S.num_pnodes > 0;
     for (int k = 0; k < 3; ++k) {
       d.cam.eye[k] = jobs[j].cam.eye[k];
       d.cam.llc[k] = jobs[j].cam.lower_left[k];
@@ -637,6 +683,10 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
     d.minb = jobs[j].minb;
     d.maxb = jobs[j].maxb;
     d.accum = jobs[j].accum;
+  }
+  if (prims != c->prims) {
+    c->prims = prims;
+    c->table_gen++;   // the captured graph holds the other traversal variants
   }
   hipStream_t s = c->stream;
   HIP_OR_FAIL(c, hipStreamSynchronize(s));

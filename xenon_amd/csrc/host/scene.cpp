@@ -244,6 +244,81 @@ void build_blas(HostMesh& m) {
   build_tree<12, 2, true>(items, m.nodes, m.prims);
 }
 
+namespace {
+
+// AABB of an analytic primitive, as the primitives' aabb() compute it
+// (geometry/primitives.h:96-99, 129-145, 204-206, 254-265, 308-317).
+Aabb prim_box(const xnh_prim& pr) {
+  const float* q = pr.p;
+  Aabb b;
+  switch (pr.type) {
+    case XNH_PRIM_SPHERE: {
+      const V3 c{q[0], q[1], q[2]}, r{q[3], q[3], q[3]};
+      b.lo = c - r;
+      b.hi = c + r;
+      break;
+    }
+    case XNH_PRIM_PLANE: {
+      const float kHuge = 1e4f, off = q[3];
+      const V3 n{q[0], q[1], q[2]};
+      const V3 a = n * (off - 0.01f), c = n * (off + 0.01f);
+      float lo[3] = {-kHuge + a.x, -kHuge + a.y, -kHuge + a.z};
+      float hi[3] = {kHuge + c.x, kHuge + c.y, kHuge + c.z};
+      for (int i = 0; i < 3; ++i) {
+        if (std::abs(q[i]) > 0.9f) {
+          lo[i] = off - 0.01f;
+          hi[i] = off + 0.01f;
+        }
+      }
+      b.lo = {lo[0], lo[1], lo[2]};
+      b.hi = {hi[0], hi[1], hi[2]};
+      break;
+    }
+    case XNH_PRIM_BOX: {
+      const V3 c{q[0], q[1], q[2]}, h{q[3], q[4], q[5]};
+      b.lo = c - h;
+      b.hi = c + h;
+      break;
+    }
+    case XNH_PRIM_DISK: {
+      const V3 c{q[0], q[1], q[2]};
+      const float r = q[6];
+      float e[3] = {r * std::sqrt(1.f - q[3] * q[3]), r * std::sqrt(1.f - q[4] * q[4]),
+                    r * std::sqrt(1.f - q[5] * q[5])};
+      for (float& v : e) v = keep_max(v, 0.001f);   // max3(e, Vec3(0.001f))
+      b.lo = c - V3{e[0], e[1], e[2]};
+      b.hi = c + V3{e[0], e[1], e[2]};
+      break;
+    }
+    default: {   // XNH_PRIM_QUAD
+      const V3 o{q[0], q[1], q[2]}, u{q[3], q[4], q[5]}, v{q[6], q[7], q[8]};
+      b.add(o);
+      b.add(o + u);
+      b.add(o + v);
+      b.add(o + u + v);
+      b.lo = b.lo - V3{0.001f, 0.001f, 0.001f};
+      b.hi = b.hi + V3{0.001f, 0.001f, 0.001f};
+      break;
+    }
+  }
+  return b;
+}
+
+}  // namespace
+
+// PrimBLAS::build (geometry/prim_blas.cpp:142-160, 34-118): the binned SAH
+// of the triangle BLAS over the primitives' boxes.
+void build_prims(HostScene& s) {
+  const uint32_t n = static_cast<uint32_t>(s.prims.size());
+  std::vector<Item> items(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    items[i].box = prim_box(s.prims[i]);
+    items[i].c = items[i].box.center();
+    items[i].id = i;
+  }
+  build_tree<12, 2, true>(items, s.prim_nodes, s.prim_order);
+}
+
 // geometry/transform.h:105-148: uniform scale, Euler Y then X then Z.
 xnh_instance make_instance(int32_t mesh, int32_t id, const float t[3], float s, const float rot_deg[3]) {
   xnh_instance in{};
@@ -288,14 +363,26 @@ void build_tlas(HostScene& s) {
   } else {
     for (const auto& p : s.pending) s.instances.push_back(make_instance(p.mesh, p.id, p.t, p.s, p.rot));
   }
+  // the primitives: one identity instance after the others
+  // (Scene::build_acceleration, scene/scene.h:97-110)
+  if (!s.prims.empty()) {
+    xnh_instance in{};
+    in.mesh = XNH_PRIM_GEOMETRY;
+    in.instance_id = static_cast<int32_t>(s.instances.size());
+    const float id[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    std::memcpy(in.local_to_world, id, sizeof id);
+    std::memcpy(in.world_to_local, id, sizeof id);
+    s.instances.push_back(in);
+  }
   const uint32_t n = static_cast<uint32_t>(s.instances.size());
   std::vector<Item> items(n);
   for (uint32_t i = 0; i < n; ++i) {
-    const HostMesh& m = s.meshes[s.instances[i].mesh];
+    const std::vector<xnh_bvh_node>& nodes =
+        s.instances[i].mesh == XNH_PRIM_GEOMETRY ? s.prim_nodes : s.meshes[s.instances[i].mesh].nodes;
     Aabb local;
-    if (!m.nodes.empty()) {
-      local.lo = {m.nodes[0].lo[0], m.nodes[0].lo[1], m.nodes[0].lo[2]};
-      local.hi = {m.nodes[0].hi[0], m.nodes[0].hi[1], m.nodes[0].hi[2]};
+    if (!nodes.empty()) {
+      local.lo = {nodes[0].lo[0], nodes[0].lo[1], nodes[0].lo[2]};
+      local.hi = {nodes[0].hi[0], nodes[0].hi[1], nodes[0].hi[2]};
     }
     Aabb w;   // the eight transformed corners (transform.h:88-98)
     for (int c = 0; c < 8; ++c) {

@@ -50,6 +50,9 @@ struct HostScene {
   std::vector<xnh_light> lights;
   std::vector<xnh_dir_light> suns;
   xnh_sky sky;
+  std::vector<xnh_prim> prims;             // Scene::all_prims
+  std::vector<xnh_bvh_node> prim_nodes;    // its PrimBLAS
+  std::vector<uint32_t> prim_order;
   std::vector<PendingInstance> pending;
   std::vector<xnh_instance> instances;
   std::vector<xnh_bvh_node> tlas_nodes;
@@ -62,6 +65,7 @@ struct HostScene {
 xnh_material prepare_material(const float* p16);
 void smooth_normals(HostMesh& m);
 void build_blas(HostMesh& m);
+void build_prims(HostScene& s);
 void build_tlas(HostScene& s);
 xnh_instance make_instance(int32_t mesh, int32_t id, const float t[3], float s, const float rot_deg[3]);
 xnh_camera look_at(const float eye[3], const float target[3], const float up[3], float fov_deg, float aspect);

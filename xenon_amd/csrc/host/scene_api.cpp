@@ -109,6 +109,21 @@ int xnh_scene_set_sky(xnh_scene* s, const float* z, const float* h, const float*
   return XNH_OK;
 }
 
+int xnh_scene_add_primitive(xnh_scene* s, int32_t type, const float* params, int32_t mat_id) {
+  if (!s || !params || type < XNH_PRIM_SPHERE || type > XNH_PRIM_QUAD || mat_id < 0) return XNH_E_INVALID;
+  static const int kParams[5] = {4, 4, 6, 7, 9};
+  xnh_prim pr{};
+  pr.type = type;
+  pr.mat_id = mat_id;
+  for (int k = 0; k < kParams[type]; ++k) {
+    if (!std::isfinite(params[k])) return XNH_E_INVALID;
+    pr.p[k] = params[k];
+  }
+  s->s.prims.push_back(pr);
+  s->s.built = false;
+  return static_cast<int>(s->s.prims.size()) - 1;
+}
+
 int xnh_scene_add_instance(xnh_scene* s, int32_t mesh, const float* t, float scale, const float* rot, int32_t id) {
   if (!s || !t || !rot || mesh < 0) return XNH_E_INVALID;
   PendingInstance p{};
@@ -131,6 +146,7 @@ int xnh_scene_build(xnh_scene* s, int32_t nthreads) {
   for (const auto& p : hs.pending)
     if (p.mesh >= static_cast<int32_t>(hs.meshes.size())) return XNH_E_INVALID;
   for (auto& m : hs.meshes) build_blas(m);
+  build_prims(hs);
   build_tlas(hs);
   hs.mesh_views.resize(hs.meshes.size());
   for (size_t i = 0; i < hs.meshes.size(); ++i) {
@@ -171,6 +187,11 @@ int xnh_scene_flat(const xnh_scene* s, xnh_flat_scene* out) {
   out->num_dir_lights = static_cast<int32_t>(hs.suns.size());
   out->dir_lights = hs.suns.data();
   out->sky = hs.sky;
+  out->num_prims = static_cast<int32_t>(hs.prims.size());
+  out->prims = hs.prims.data();
+  out->num_prim_nodes = static_cast<int32_t>(hs.prim_nodes.size());
+  out->prim_nodes = hs.prim_nodes.data();
+  out->prim_order = hs.prim_order.data();
   return XNH_OK;
 }
 

@@ -3,8 +3,7 @@
 // (material/material.cpp:31-152) and Wavefront OBJ meshes
 // (geometry/mesh.cpp:42-151), with the same directives, defaults, path rules
 // and arithmetic, so that a file loads into the scene the reference builds
-// from it. Analytic primitives (sphere, box, disk, quad, plane) are counted
-// and skipped: the GPU path traces triangles only.
+// from it, analytic primitives (sphere, box, disk, quad, plane) included.
 //
 // Compiled with g++ -ffp-contract=off (the reference's host arithmetic).
 #include <cmath>
@@ -235,6 +234,18 @@ int add_host_mesh(xnh_scene* s, const xnh::HostMesh& m) {
                             m.mat_ids.data());
 }
 
+// n / |n| when |n| > 1e-6 (the parser's normalisation of plane and disk
+// normals; Vec3 / s multiplies by 1 / s).
+void unit_if_long(float* n) {
+  const float len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (len > 1e-6f) {
+    const float r = 1.f / len;
+    n[0] *= r;
+    n[1] *= r;
+    n[2] *= r;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -359,7 +370,34 @@ int xnh_scene_load_file(const char* path, xnh_scene** out, xnh_camera* camera, x
       rc = xnh_scene_add_instance(s, mesh, t, scale, rot, id);
       ++ninst;
     } else if (cmd == "sphere" || cmd == "box" || cmd == "disk" || cmd == "quad" || cmd == "plane") {
-      ++fi.skipped_primitives;
+      // analytic primitives (scene_file.cpp:175-233); an unknown material
+      // name selects material 0, as resolve_mat does
+      float q[10] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+      std::string mname;
+      int type = XNH_PRIM_SPHERE;
+      if (cmd == "sphere") {
+        ss >> q[0] >> q[1] >> q[2] >> q[3] >> mname;
+      } else if (cmd == "box") {
+        type = XNH_PRIM_BOX;
+        ss >> q[0] >> q[1] >> q[2] >> q[3] >> q[4] >> q[5] >> mname;
+      } else if (cmd == "disk") {
+        type = XNH_PRIM_DISK;
+        ss >> q[0] >> q[1] >> q[2] >> q[3] >> q[4] >> q[5] >> q[6] >> mname;
+        unit_if_long(q + 3);
+      } else if (cmd == "quad") {
+        type = XNH_PRIM_QUAD;
+        ss >> q[0] >> q[1] >> q[2] >> q[3] >> q[4] >> q[5] >> q[6] >> q[7] >> q[8] >> mname;
+      } else {
+        type = XNH_PRIM_PLANE;
+        ss >> q[0] >> q[1] >> q[2] >> q[3] >> mname;
+        unit_if_long(q);
+      }
+      auto it = mats.find(mname);
+      int mid = 0;
+      if (it != mats.end()) mid = it->second;
+      else ++fi.unknown_materials;
+      rc = xnh_scene_add_primitive(s, type, q, mid);
+      if (rc >= 0) ++fi.primitives;
     }
   }
   if (rc >= 0) rc = xnh_scene_build(s, 0);

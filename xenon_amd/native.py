@@ -73,6 +73,14 @@ class SkyRec(C.Structure):
                 ("sharpness", C.c_float), ("intensity", C.c_float)]
 
 
+class PrimRec(C.Structure):
+    _fields_ = [("type", C.c_int32), ("mat_id", C.c_int32), ("p", C.c_float * 10)]
+
+
+PRIM_SPHERE, PRIM_PLANE, PRIM_BOX, PRIM_DISK, PRIM_QUAD = 0, 1, 2, 3, 4
+PRIM_GEOMETRY = -1
+
+
 class FlatScene(C.Structure):
     _fields_ = [("num_meshes", C.c_int32), ("meshes", C.POINTER(MeshView)),
                 ("num_instances", C.c_int32), ("instances", C.POINTER(InstanceRec)),
@@ -80,7 +88,9 @@ class FlatScene(C.Structure):
                 ("num_materials", C.c_int32), ("materials", C.POINTER(MaterialRec)),
                 ("num_lights", C.c_int32), ("lights", C.POINTER(LightRec)),
                 ("num_dir_lights", C.c_int32), ("dir_lights", C.POINTER(DirLightRec)),
-                ("sky", SkyRec)]
+                ("sky", SkyRec),
+                ("num_prims", C.c_int32), ("prims", C.POINTER(PrimRec)),
+                ("num_prim_nodes", C.c_int32), ("prim_nodes", C.POINTER(BvhNode)), ("prim_order", _u32p)]
 
 
 class Ray(C.Structure):
@@ -126,7 +136,8 @@ class TuneResult(C.Structure):
 class FileInfo(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("samples", C.c_int32),
                 ("min_bounces", C.c_int32), ("max_bounces", C.c_int32), ("meshes", C.c_int32),
-                ("skipped_primitives", C.c_int32), ("missing_files", C.c_int32), ("instances", C.c_int32),
+                ("primitives", C.c_int32), ("missing_files", C.c_int32), ("instances", C.c_int32),
+                ("unknown_materials", C.c_int32),
                 ("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float)]
 
 
@@ -155,6 +166,7 @@ SIGNATURES = {
     "xnh_scene_add_dir_light": (C.c_int, [_vp, _fp, _fp, C.c_float]),
     "xnh_scene_set_sky": (C.c_int, [_vp, _fp, _fp, _fp, C.c_float, C.c_float]),
     "xnh_scene_add_instance": (C.c_int, [_vp, C.c_int32, _fp, C.c_float, _fp, C.c_int32]),
+    "xnh_scene_add_primitive": (C.c_int, [_vp, C.c_int32, _fp, C.c_int32]),
     "xnh_scene_build": (C.c_int, [_vp, C.c_int32]),
     "xnh_scene_flat": (C.c_int, [_vp, C.POINTER(FlatScene)]),
     "xnh_camera_look_at": (C.c_int, [_fp, _fp, _fp, C.c_float, C.c_float, C.POINTER(Camera)]),

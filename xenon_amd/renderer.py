@@ -59,6 +59,10 @@ class HipScene:
         sk = desc.sky
         z, hz, g = (np.array(v, np.float32) for v in (sk.zenith, sk.horizon, sk.ground))
         N.check(L.xnh_scene_set_sky(self.h, _fp(z), _fp(hz), _fp(g), sk.intensity, sk.sharpness))
+        for pr in desc.prims:
+            q = np.zeros(10, np.float32)
+            q[:len(pr.params)] = pr.params
+            N.check(L.xnh_scene_add_primitive(self.h, pr.type_id, _fp(q), pr.mat))
         for ins in desc.instances:
             t = np.array(ins.translation, np.float32)
             r = np.array(ins.rot_deg, np.float32)
@@ -71,8 +75,9 @@ class HipScene:
     def from_file(cls, path: str) -> "HipScene":
         """A .xenon scene file (load_scene, scene/scene_file.cpp:49-240) with
         its .mat materials and OBJ meshes. ``info`` holds the file's render
-        settings and what was skipped (analytic primitives, files that did not
-        open); ``camera()`` uses the file's camera."""
+        settings and counts (meshes, analytic primitives, instances, files
+        that did not open, unknown material names); ``camera()`` uses the
+        file's camera."""
         self = cls.__new__(cls)
         self.desc = None
         h = C.c_void_p()
@@ -152,6 +157,28 @@ class HipScene:
             meshes[i] = r.mesh
         return nodes, order, xf, ids, meshes
 
+
+    def prims(self):
+        """The analytic primitives as stored: ((type, mat_id) per primitive,
+        10 parameters per primitive in xnh_prim order)."""
+        f = self._flat
+        tm = np.zeros((f.num_prims, 2), np.int32)
+        p = np.zeros((f.num_prims, 10), np.float32)
+        for i in range(f.num_prims):
+            r = f.prims[i]
+            tm[i] = (r.type, r.mat_id)
+            p[i] = r.p[:]
+        return tm, p
+
+    def prim_blas(self):
+        """The primitives' BVH: (nodes, leaf slot -> primitive index)."""
+        f = self._flat
+        if f.num_prim_nodes == 0:
+            return np.zeros(0, NODE_DTYPE), np.zeros(0, np.uint32)
+        nodes = np.ctypeslib.as_array(C.cast(f.prim_nodes, C.POINTER(C.c_uint8)), shape=(f.num_prim_nodes * 32,))
+        nodes = nodes.view(NODE_DTYPE).copy()
+        order = np.ctypeslib.as_array(f.prim_order, shape=(f.num_prims,)).copy()
+        return nodes, order
 
     def bounds(self):
         """World bounding box of the scene (the TLAS root box): (lo, hi)."""

@@ -96,6 +96,23 @@ class Instance:
 
 
 @dataclass
+class Prim:
+    """Analytic primitive (geometry/primitives.h:54-318); params by kind:
+    sphere (cx, cy, cz, r), plane (nx, ny, nz, offset), box (cx, cy, cz, hx,
+    hy, hz), disk (cx, cy, cz, nx, ny, nz, r), quad (ox, oy, oz, ux, uy, uz,
+    vx, vy, vz). Plane and disk normals are used as given."""
+    kind: str
+    params: tuple
+    mat: int
+
+    KINDS = ("sphere", "plane", "box", "disk", "quad")
+
+    @property
+    def type_id(self) -> int:
+        return self.KINDS.index(self.kind)
+
+
+@dataclass
 class Camera:
     eye: tuple
     target: tuple
@@ -118,6 +135,7 @@ class SceneDesc:
     suns: list = field(default_factory=list)
     sky: Sky = field(default_factory=Sky)
     instances: list = field(default_factory=list)
+    prims: list = field(default_factory=list)
 
     @property
     def aspect(self) -> float:
@@ -523,6 +541,56 @@ def caustic_room(width=1280, height=720, spp=64, freq=158, max_bounces=8, min_bo
     sky = Sky(intensity=0.0)
     return SceneDesc("caustic_room", [room, glass], materials, lights, cam, width, height, spp,
                      min_bounces=min_bounces, max_bounces=max_bounces, sky=sky)
+
+
+# ---------------------------------------------------------------------------
+# analytic primitives beside meshes
+# ---------------------------------------------------------------------------
+
+def prim_shapes(width=800, height=450, spp=64, n_spheres=24, max_bounces=8, min_bounces=3, seed=5) -> SceneDesc:
+    """The five primitive kinds of the reference's prim_types scene (ground
+    plane, sphere, box, disk, tilted quad, thin metal box) plus a glass
+    sphere and `n_spheres` small random spheres resting on the ground, next
+    to two meshes: an emissive quad overhead (two area-light triangles,
+    facing down) and a glossy icosphere. Sun and sky are on, so rays end on
+    primitives, meshes, the light and the sky."""
+    rng = np.random.default_rng(seed)
+    PLANE, RED, BLUE, YELLOW, GREEN, METAL, GLASS, LIGHTM, GLOSSY = range(9)
+    materials = [Material(base=(0.72, 0.70, 0.68), roughness=0.8),
+                 Material(base=(0.85, 0.12, 0.08), roughness=0.3),
+                 Material(base=(0.10, 0.45, 0.80), roughness=0.5),
+                 Material(base=(0.90, 0.75, 0.10), roughness=0.4),
+                 Material(base=(0.15, 0.80, 0.25), roughness=0.5),
+                 Material(base=(0.90, 0.90, 0.90), roughness=0.05, metallic=1.0),
+                 Material(base=(1.0, 1.0, 1.0), roughness=0.0, ior=1.5, transmission=1.0),
+                 Material(base=(0.8, 0.8, 0.8), roughness=1.0, ior=1.0, emission=(14.0, 12.0, 9.0)),
+                 Material(base=(0.95, 0.64, 0.54), roughness=0.25, metallic=1.0)]
+    dn = np.array([0.0, 0.5, 1.0], F32)
+    dn = dn / np.sqrt(np.float32(dn @ dn))
+    prims = [Prim("plane", (0.0, 1.0, 0.0, 0.0), PLANE),
+             Prim("sphere", (-7.0, 1.4, 0.0, 1.4), RED),
+             Prim("box", (-3.5, 1.2, 0.0, 1.2, 1.2, 1.2), BLUE),
+             Prim("disk", (0.0, 1.8, 0.5, *map(float, dn), 1.3), YELLOW),
+             Prim("quad", (2.8, 0.0, -0.8, 2.4, 0.0, 0.0, 0.0, 2.8, -0.8), GREEN),
+             Prim("box", (7.0, 0.5, 0.0, 1.6, 0.5, 1.0), METAL),
+             Prim("sphere", (-1.6, 0.9, 3.2, 0.9), GLASS)]
+    for _ in range(n_spheres):
+        r = float(rng.uniform(0.15, 0.45))
+        x, z = float(rng.uniform(-9.0, 9.0)), float(rng.uniform(-5.0, 6.0))
+        prims.append(Prim("sphere", (x, r, z, r), int(rng.choice([RED, BLUE, YELLOW, GREEN, METAL, GLASS]))))
+    verts, tris = [], []
+    _quad(verts, tris, (-1.5, 7.0, -1.0), (1.5, 7.0, -1.0), (1.5, 7.0, 1.0), (-1.5, 7.0, 1.0), facing=(0, -1, 0))
+    lamp = Mesh(np.array(verts, F32), np.array(tris, np.int32), np.full(len(tris), LIGHTM, np.int32))
+    V, F = icosphere(2)
+    ball = Mesh((V * 0.9 + np.array([4.6, 0.9, 2.6])).astype(F32), F.astype(np.int32),
+                np.full(F.shape[0], GLOSSY, np.int32))
+    P = lamp.positions
+    lights = [Light(0, t, (14.0, 12.0, 9.0), _tri_area(*P[lamp.indices[t]])) for t in range(2)]
+    cam = Camera(eye=(0.0, 5.0, 20.0), target=(0.0, 2.0, 0.0), fov=50.0)
+    return SceneDesc("prim_shapes", [lamp, ball], materials, lights, cam, width, height, spp,
+                     min_bounces=min_bounces, max_bounces=max_bounces,
+                     suns=[Sun((-0.4, 1.0, 0.3), (2.0, 1.9, 1.7), 1.0)],
+                     sky=Sky((0.15, 0.25, 0.55), (0.65, 0.72, 0.80), (0.15, 0.12, 0.10), 0.8, 3.0), prims=prims)
 
 
 # ---------------------------------------------------------------------------
