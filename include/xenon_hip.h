@@ -313,6 +313,27 @@ int xnh_get_tuning(const xnh_ctx* ctx, int32_t* block_size, int32_t* lds_stack, 
  * accumulation. */
 int xnh_autotune(xnh_ctx* ctx, const xnh_camera* camera, int32_t probe_spp, xnh_tune_result* out);
 
+/* ---- output stage (main.cpp:25-52, math/image.h:11-182) ----------------
+ * Both calls take a host framebuffer of W*H*3 floats in the layout the
+ * reference's save_image receives (the swapchain buffer: rows top first),
+ * or rgb = NULL for this context's running average in that layout (what
+ * render_frame writes; W and H must then be the context's). */
+typedef struct {           /* NoiseStats (math/image.h:11-31) */
+  float mean_squared_score, rms_score, mean_absolute, median_absolute, p95_absolute, max_absolute;
+  float avg_luminance, edge_threshold, epsilon;
+  int32_t valid_pixel_count, total_interior_pixel_count;
+  float valid_pixel_ratio;
+} xnh_noise_stats;
+/* ImageNoiseEstimator::computeResidualNoiseStats (image.h:35-118) on the
+ * device. Images of width or height <= 2 give the all-zero result, as there.
+ * At most 2^26 pixels. */
+int xnh_image_noise_stats(xnh_ctx* ctx, const float* rgb, int32_t width, int32_t height, xnh_noise_stats* out);
+/* The 8-bit pixels save_image passes to stbi_write_png (main.cpp:26-31):
+ * (uint8)(255.99 * clamp(c, 0, 1)^(1/2.2)), W*H*3 bytes in the file's row
+ * order (the writer flips the framebuffer vertically, so file row r is
+ * framebuffer row H-1-r). */
+int xnh_image_rgb8(xnh_ctx* ctx, const float* rgb, int32_t width, int32_t height, uint8_t* out);
+
 /* Copies the raw per-pixel sums (W*H*3) and returns the sample count. */
 int xnh_read_accum(xnh_ctx* ctx, float* sum_out, int32_t* spp_out);
 int xnh_reset(xnh_ctx* ctx);

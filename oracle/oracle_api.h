@@ -54,6 +54,16 @@ void orc_pcg_stream(uint64_t seed, uint64_t seq, int n, uint32_t* out);
 void orc_bsdf_sample(const float* params16, const float* wo3, uint64_t seed, int n, float* out9);
 void orc_bsdf_eval(const float* params16, int n, const float* wo3, const float* wi3, float* out4);
 
+/* NoiseStats of a W x H x 3 framebuffer in the layout save_image receives
+ * (math/image.h:35-118). out9: meanSquaredScore, rmsScore, meanAbsolute,
+ * medianAbsolute, p95Absolute, maxAbsolute, avgLuminance, edgeThreshold,
+ * epsilon; out3: validPixelCount, totalInteriorPixelCount, and
+ * validPixelRatio as float bits. */
+void orc_noise_stats(const float* rgb, int w, int h, float* out9, int* out3);
+/* The 8-bit pixels save_image hands to the PNG writer (main.cpp:26-31),
+ * rows in file order (the writer flips the framebuffer vertically). */
+void orc_rgb8(const float* rgb, int w, int h, uint8_t* out);
+
 #ifdef __cplusplus
 }
 #endif

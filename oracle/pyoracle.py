@@ -55,7 +55,12 @@ _SIGS = {
     "orc_pcg_stream": (None, [C.c_uint64, C.c_uint64, C.c_int, _u32p]),
     "orc_bsdf_sample": (None, [_fp, _fp, C.c_uint64, C.c_int, _fp]),
     "orc_bsdf_eval": (None, [_fp, C.c_int, _fp, _fp, _fp]),
+    "orc_noise_stats": (None, [_fp, C.c_int, C.c_int, _fp, _ip]),
+    "orc_rgb8": (None, [_fp, C.c_int, C.c_int, _u8p]),
 }
+
+NOISE_FIELDS = ("mean_squared_score", "rms_score", "mean_absolute", "median_absolute", "p95_absolute",
+                "max_absolute", "avg_luminance", "edge_threshold", "epsilon")
 
 NODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("child", "<u4"), ("count", "<u4")])
 
@@ -108,6 +113,26 @@ class Lib:
         p = _arr(params16, np.float32)
         w = _arr(wo, np.float32)
         self.dll.orc_bsdf_sample(p.ctypes.data_as(_fp), w.ctypes.data_as(_fp), seed, n, out.ctypes.data_as(_fp))
+        return out
+
+    def noise_stats(self, rgb, width: int, height: int) -> dict:
+        """NoiseStats of a framebuffer (W*H*3 floats, the layout save_image
+        receives): the nine float fields by name plus valid_pixel_count,
+        total_interior_pixel_count and valid_pixel_ratio."""
+        img = _arr(rgb, np.float32).reshape(-1)
+        o = np.zeros(9, np.float32)
+        oi = np.zeros(3, np.int32)
+        self.dll.orc_noise_stats(img.ctypes.data_as(_fp), width, height, o.ctypes.data_as(_fp), oi.ctypes.data_as(_ip))
+        out = {k: np.float32(v) for k, v in zip(NOISE_FIELDS, o)}
+        out.update(valid_pixel_count=int(oi[0]), total_interior_pixel_count=int(oi[1]),
+                   valid_pixel_ratio=oi[2:3].view(np.float32)[0])
+        return out
+
+    def rgb8(self, rgb, width: int, height: int) -> np.ndarray:
+        """The 8-bit pixels save_image writes, (H, W, 3) in file row order."""
+        img = _arr(rgb, np.float32).reshape(-1)
+        out = np.zeros((height, width, 3), np.uint8)
+        self.dll.orc_rgb8(img.ctypes.data_as(_fp), width, height, out.ctypes.data_as(_u8p))
         return out
 
     def bsdf_eval(self, params16, wo, wi) -> np.ndarray:

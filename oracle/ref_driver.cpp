@@ -9,6 +9,8 @@
 // compare the oracle restatement against the reference byte for byte.
 // Nothing here is reference source; it only calls the reference's public
 // interfaces.
+#include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -18,6 +20,7 @@
 #include "camera/sampler.h"
 #include "material/bsdf.h"
 #include "material/material.h"
+#include "math/image.h"
 #include "render/swapchain.h"
 #include "render/wavefront.h"
 #include "scene/scene.h"
@@ -288,6 +291,27 @@ void orc_bsdf_eval(const float* p, int n, const float* wo, const float* wi, floa
     out[4 * i] = f.x; out[4 * i + 1] = f.y; out[4 * i + 2] = f.z;
     out[4 * i + 3] = xn::bsdf_pdf_for_nee(a, b, m);
   }
+}
+
+// NoiseStats: the reference's own estimator (math/image.h, header only).
+void orc_noise_stats(const float* rgb, int w, int h, float* o, int* oi) {
+  const xn::NoiseStats s = xn::ImageNoiseEstimator::computeResidualNoiseStats(rgb, w, h);
+  const float v[9] = {s.meanSquaredScore, s.rmsScore, s.meanAbsolute, s.medianAbsolute, s.p95Absolute,
+                      s.maxAbsolute, s.avgLuminance, s.edgeThreshold, s.epsilon};
+  std::memcpy(o, v, sizeof v);
+  oi[0] = s.validPixelCount;
+  oi[1] = s.totalInteriorPixelCount;
+  std::memcpy(&oi[2], &s.validPixelRatio, 4);
+}
+
+// save_image's conversion (main.cpp:26-31) restated: main.cpp itself does
+// not build without GLFW and stb. File row r is framebuffer row h-1-r.
+void orc_rgb8(const float* rgb, int w, int h, uint8_t* out) {
+  for (int r = 0; r < h; ++r)
+    for (int k = 0; k < 3 * w; ++k) {
+      const float c = std::pow(std::clamp(rgb[3 * static_cast<size_t>(h - 1 - r) * w + k], 0.f, 1.f), 1.f / 2.2f);
+      out[3 * static_cast<size_t>(r) * w + k] = static_cast<uint8_t>(255.99f * c);
+    }
 }
 
 // ---- analytic primitives (reference only; not part of oracle_api.h) ------

@@ -4,7 +4,8 @@ sources by oracle/Makefile). Run in a container that has the reference:
 
     make -C oracle ref && python tests/golden/make_golden.py [name ...]
 
-(no names: every fixture). Each fixture holds the inputs (a scene named by generator + arguments and
+(no names: every fixture; image_metrics: the output-stage fixture). Each
+scene fixture holds the inputs (a scene named by generator + arguments and
 its array digest, a ray batch) and the reference's outputs (closest hits,
 occlusion, BLAS/TLAS digests, a small averaged image).
 """
@@ -73,8 +74,50 @@ def tree_digest(osc, desc) -> str:
     return h.hexdigest()
 
 
+def image_set():
+    """Framebuffers (swapchain layout) for the output-stage fixture: smooth
+    shading with noise, hard edges, a constant field, HDR values beyond
+    [0, 1] and negatives, the smallest measured size and a degenerate one."""
+    rng = np.random.default_rng(21)
+    out = []
+    h, w = 48, 64
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float32)
+    smooth = (0.35 + 0.2 * np.sin(xx / 6.0) * np.cos(yy / 9.0))[..., None] * np.array([1.0, 0.85, 0.7], np.float32)
+    out.append(smooth + rng.normal(size=(h, w, 3)).astype(np.float32) * 0.02)
+    h, w = 60, 80
+    yy, xx = np.mgrid[0:h, 0:w]
+    blocks = (((yy // 10) + (xx // 10)) % 2).astype(np.float32)[..., None] * np.array([0.8, 0.5, 0.3], np.float32)
+    out.append(blocks + 0.05 + rng.normal(size=(h, w, 3)).astype(np.float32) * 0.03)
+    out.append(np.full((16, 16, 3), 0.42, np.float32))
+    hdr = rng.lognormal(mean=-0.5, sigma=1.2, size=(40, 50, 3)).astype(np.float32)
+    hdr[rng.random((40, 50)) < 0.05] *= -1.0
+    out.append(hdr)
+    out.append(rng.random((3, 3, 3)).astype(np.float32))
+    out.append(rng.random((2, 5, 3)).astype(np.float32))
+    return [np.ascontiguousarray(a, np.float32) for a in out]
+
+
+def write_image_fixture(ref):
+    arrs = {}
+    for k, img in enumerate(image_set()):
+        h, w = img.shape[:2]
+        st = ref.noise_stats(img, w, h)
+        arrs[f"img{k}"] = img
+        arrs[f"stats{k}"] = np.array([st[f] for f in pyoracle.NOISE_FIELDS], np.float32)
+        arrs[f"counts{k}"] = np.array([st["valid_pixel_count"], st["total_interior_pixel_count"]], np.int32)
+        arrs[f"ratio{k}"] = np.array([st["valid_pixel_ratio"]], np.float32)
+        arrs[f"rgb8_{k}"] = ref.rgb8(img, w, h)
+        print("image", k, w, h, st["valid_pixel_count"], st["rms_score"])
+    np.savez_compressed(os.path.join(HERE, "image_metrics.npz"), **arrs)
+
+
 def main(names):
     ref = pyoracle.Lib("ref")
+    if not names or "image_metrics" in names:
+        write_image_fixture(ref)
+        names = [n for n in names if n != "image_metrics"]
+        if not names and len(sys.argv) > 1:
+            return
     for name in names or SCENES:
         desc = make_scene(name)
         osc = ref.load(desc)

@@ -8,5 +8,7 @@ and the seeded workload generators in ``scenes``.
 from . import scenes  # noqa: F401
 from .native import XnhError  # noqa: F401
 from .renderer import HIT_DTYPE, NODE_DTYPE, GpuScene, HipRenderer, HipScene  # noqa: F401
+from .swapchain import TripleSwapchain, progressive  # noqa: F401
 
-__all__ = ["GpuScene", "HipRenderer", "HipScene", "XnhError", "scenes", "HIT_DTYPE", "NODE_DTYPE"]
+__all__ = ["GpuScene", "HipRenderer", "HipScene", "XnhError", "scenes", "HIT_DTYPE", "NODE_DTYPE",
+           "TripleSwapchain", "progressive"]

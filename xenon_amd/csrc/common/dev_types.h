@@ -172,4 +172,25 @@ enum Stat : int {
   kNumStat
 };
 
+// Output stage (image.hip): the device state of one NoiseStats computation
+// (math/image.h:11-31 plus the radix-select state) and its scratch.
+constexpr int kImgMaxBlocks = 2048;                 // partial-sum blocks of one pass
+constexpr int64_t kImgMaxPixels = int64_t(1) << 26; // 8192 x 8192
+struct ImgState {
+  float avg, edge, eps;        // average luminance, edge threshold, epsilon
+  float max_abs, mean_sq, mean_abs;
+  uint32_t count, pad;         // counted (interior, non-edge) pixels
+  uint32_t rank[2];            // median, 95th percentile: rank left inside the prefix
+  uint32_t prefix[2];          // bits of the selected key found so far
+  uint32_t hist[512];          // [target][digit] of the current pass
+};
+struct ImgScratch {
+  float* lum;                  // W*H luminance
+  uint32_t* key;               // W*H residual keys
+  double* part;                // 2 * kImgMaxBlocks partial sums
+  float* pmax;                 // kImgMaxBlocks partial maxima
+  uint32_t* pcnt;              // kImgMaxBlocks partial counts
+  ImgState* st;
+};
+
 }  // namespace xnh

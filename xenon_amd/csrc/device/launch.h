@@ -54,4 +54,10 @@ hipError_t launch_guide_collect(hipStream_t s, const WaveBuffers& B, float* lam_
                                 float* lam_sum, uint32_t* n_out, uint32_t cap);
 hipError_t launch_to_half(hipStream_t s, const float* src, uint16_t* dst, size_t count);
 
+// Output stage (image.hip). img: W*H*3 floats in the framebuffer layout the
+// PNG writer receives (rows top first).
+hipError_t launch_img_from_accum(hipStream_t s, const float4* accum, float inv_spp, int w, int h, float* img);
+hipError_t launch_noise_stats(hipStream_t s, const float* img, int w, int h, ImgScratch sc);
+hipError_t launch_img_rgb8(hipStream_t s, const float* img, int w, int h, uint8_t* out);
+
 }  // namespace xnh

@@ -103,6 +103,13 @@ struct xnh_ctx {
   int grid_shade = 0;
   float* d_out = nullptr;
   int spp_total = 0;
+  struct {                        // output-stage scratch (xnh_image_noise_stats, xnh_image_rgb8)
+    int64_t cap = 0;              // pixels
+    float* img = nullptr;         // W*H*3 framebuffer
+    uint8_t* rgb8 = nullptr;
+    ImgScratch sc{};
+    std::vector<void*> allocs;
+  } out;
   xnh_stats stats{};
 };
 
@@ -825,6 +832,7 @@ void xnh_destroy(xnh_ctx* c) {
   free_all(c->own.allocs);
   free_all(c->wave_allocs);
   free_all(c->guide.allocs);
+  free_all(c->out.allocs);
   if (c->d_jobs) hipFree(c->d_jobs);
   if (c->d_chunks) hipFree(c->d_chunks);
   if (c->d_nchunks) hipFree(c->d_nchunks);
@@ -954,6 +962,88 @@ int xnh_render_jobs(xnh_ctx* c, const xnh_job* jobs, int32_t njobs) {
   hipStreamSynchronize(c->stream);
   free_all(tmp);
   return rc;
+}
+
+namespace {
+
+int ensure_out(xnh_ctx* c, int64_t npix) {
+  if (npix <= c->out.cap) return XNH_OK;
+  free_all(c->out.allocs);
+  c->out.cap = 0;
+  auto& o = c->out;
+  int rc = dev_alloc(c, o.allocs, &o.img, static_cast<size_t>(3 * npix));
+  if (rc == XNH_OK) rc = dev_alloc(c, o.allocs, &o.rgb8, static_cast<size_t>(3 * npix));
+  if (rc == XNH_OK) rc = dev_alloc(c, o.allocs, &o.sc.lum, static_cast<size_t>(npix));
+  if (rc == XNH_OK) rc = dev_alloc(c, o.allocs, &o.sc.key, static_cast<size_t>(npix));
+  if (rc == XNH_OK) rc = dev_alloc(c, o.allocs, &o.sc.part, static_cast<size_t>(2 * kImgMaxBlocks));
+  if (rc == XNH_OK) rc = dev_alloc(c, o.allocs, &o.sc.pmax, static_cast<size_t>(kImgMaxBlocks));
+  if (rc == XNH_OK) rc = dev_alloc(c, o.allocs, &o.sc.pcnt, static_cast<size_t>(kImgMaxBlocks));
+  if (rc == XNH_OK) rc = dev_alloc(c, o.allocs, &o.sc.st, 1);
+  if (rc != XNH_OK) {
+    free_all(o.allocs);
+    return rc;
+  }
+  o.cap = npix;
+  return XNH_OK;
+}
+
+// The framebuffer of an output-stage call in c->out.img: the caller's host
+// image, or the running average in the swapchain layout.
+int output_source(xnh_ctx* c, const float* rgb, int w, int h) {
+  const int64_t npix = static_cast<int64_t>(w) * h;
+  if (npix > kImgMaxPixels) return fail(c, XNH_E_INVALID, "image larger than 2^26 pixels");
+  if (!rgb && (w != c->cfg.width || h != c->cfg.height))
+    return fail(c, XNH_E_INVALID, "the running average has the context's size");
+  const int rc = ensure_out(c, npix);
+  if (rc != XNH_OK) return rc;
+  if (rgb) {
+    HIP_OR_FAIL(c, hipMemcpyAsync(c->out.img, rgb, sizeof(float) * 3 * npix, hipMemcpyHostToDevice, c->stream));
+  } else {
+    const float inv = c->spp_total > 0 ? 1.0f / static_cast<float>(c->spp_total) : 0.f;
+    HIP_OR_FAIL(c, launch_img_from_accum(c->stream, c->accum, inv, w, h, c->out.img));
+  }
+  return XNH_OK;
+}
+
+}  // namespace
+
+int xnh_image_noise_stats(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, xnh_noise_stats* out) {
+  if (!c || !out || w < 0 || h < 0) return XNH_E_INVALID;
+  *out = xnh_noise_stats{};
+  if (w <= 2 || h <= 2) return XNH_OK;   // image.h:41-42
+  int rc = output_source(c, rgb, w, h);
+  if (rc != XNH_OK) return rc;
+  HIP_OR_FAIL(c, launch_noise_stats(c->stream, c->out.img, w, h, c->out.sc));
+  ImgState st;
+  HIP_OR_FAIL(c, hipMemcpyAsync(&st, c->out.sc.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  const int interior = (w - 2) * (h - 2);
+  out->total_interior_pixel_count = interior;
+  out->avg_luminance = st.avg;
+  out->edge_threshold = st.edge;
+  out->epsilon = st.eps;
+  out->valid_pixel_count = static_cast<int32_t>(st.count);
+  out->valid_pixel_ratio = static_cast<float>(st.count) / static_cast<float>(interior);
+  out->max_absolute = st.max_abs;
+  if (st.count > 0) {   // image.h:108-115
+    out->mean_squared_score = st.mean_sq;
+    out->rms_score = std::sqrt(st.mean_sq);
+    out->mean_absolute = st.mean_abs;
+    std::memcpy(&out->median_absolute, &st.prefix[0], 4);
+    std::memcpy(&out->p95_absolute, &st.prefix[1], 4);
+  }
+  return XNH_OK;
+}
+
+int xnh_image_rgb8(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, uint8_t* out) {
+  if (!c || !out || w < 0 || h < 0) return XNH_E_INVALID;
+  if (w == 0 || h == 0) return XNH_OK;
+  int rc = output_source(c, rgb, w, h);
+  if (rc != XNH_OK) return rc;
+  HIP_OR_FAIL(c, launch_img_rgb8(c->stream, c->out.img, w, h, c->out.rgb8));
+  HIP_OR_FAIL(c, hipMemcpyAsync(out, c->out.rgb8, static_cast<size_t>(3) * w * h, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
+  return XNH_OK;
 }
 
 int xnh_read_accum(xnh_ctx* c, float* sum_out, int32_t* spp_out) {

@@ -141,6 +141,18 @@ class FileInfo(C.Structure):
                 ("eye", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float)]
 
 
+class NoiseStats(C.Structure):
+    _fields_ = [("mean_squared_score", C.c_float), ("rms_score", C.c_float), ("mean_absolute", C.c_float),
+                ("median_absolute", C.c_float), ("p95_absolute", C.c_float), ("max_absolute", C.c_float),
+                ("avg_luminance", C.c_float), ("edge_threshold", C.c_float), ("epsilon", C.c_float),
+                ("valid_pixel_count", C.c_int32), ("total_interior_pixel_count", C.c_int32),
+                ("valid_pixel_ratio", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_u8p = C.POINTER(C.c_uint8)
 _vp = C.c_void_p
 SIGNATURES = {
     "xnh_create": (C.c_int, [C.POINTER(Config), C.POINTER(_vp)]),
@@ -149,6 +161,8 @@ SIGNATURES = {
     "xnh_upload_scene": (C.c_int, [_vp, C.POINTER(FlatScene)]),
     "xnh_render": (C.c_int, [_vp, C.POINTER(Camera), C.c_int32, C.c_int32, _fp]),
     "xnh_read_accum": (C.c_int, [_vp, _fp, _ip]),
+    "xnh_image_noise_stats": (C.c_int, [_vp, _fp, C.c_int32, C.c_int32, C.POINTER(NoiseStats)]),
+    "xnh_image_rgb8": (C.c_int, [_vp, _fp, C.c_int32, C.c_int32, _u8p]),
     "xnh_reset": (C.c_int, [_vp]),
     "xnh_spp": (C.c_int, [_vp]),
     "xnh_get_stats": (C.c_int, [_vp, C.POINTER(Stats)]),

@@ -241,12 +241,20 @@ class HipRenderer:
         self._chk(N.lib().xnh_upload_scene(self.h, C.byref(scene.flat)))
         self.scene = scene
 
-    def render_frame(self, camera: N.Camera) -> np.ndarray:
+    def render_frame(self, camera: N.Camera, chain=None) -> np.ndarray:
         """Adds exactly one sample (index spp()+1) and returns the running
         average as the reference's swapchain holds it: (H, W, 3), row 0 at the
-        top (rows flipped, render/cpu_tiled.cpp:228-229)."""
+        top (rows flipped, render/cpu_tiled.cpp:228-229). With a
+        TripleSwapchain the frame goes into its write buffer, which is then
+        published (cuda_renderer.cu:501-590)."""
         img = self.render(camera, self.spp() + 1, 1)
-        return img.reshape(self.height, self.width, 3)[::-1].copy()
+        frame = img.reshape(self.height, self.width, 3)[::-1]
+        if chain is None:
+            return frame.copy()
+        out = chain.get_write_buffer()
+        np.copyto(out, frame)
+        chain.swap_writer()
+        return out
 
     def spp(self) -> int:
         return N.lib().xnh_spp(self.h)
@@ -259,6 +267,56 @@ class HipRenderer:
         self._chk(N.lib().xnh_render(self.h, C.byref(camera), int(spp_first), int(spp_count),
                                      _fp(out) if out is not None else None))
         return out
+
+    # -- output stage (main.cpp:25-52, math/image.h) ---------------------
+    def _framebuffer(self, rgb, width, height):
+        if rgb is None:
+            return None, self.width, self.height
+        a = np.ascontiguousarray(rgb, np.float32)
+        if width is None:
+            if a.ndim != 3:
+                raise ValueError("give width and height for a flat framebuffer")
+            height, width = a.shape[:2]
+        if a.size != width * height * 3:
+            raise ValueError("framebuffer size does not match width x height x 3")
+        return a, width, height
+
+    def noise_stats(self, rgb=None, width=None, height=None) -> dict:
+        """NoiseStats (math/image.h:35-118) computed on the device, of a
+        framebuffer in the swapchain layout (row 0 at the top, as
+        render_frame returns it), or of the running average when rgb is
+        None."""
+        a, w, h = self._framebuffer(rgb, width, height)
+        st = N.NoiseStats()
+        self._chk(N.lib().xnh_image_noise_stats(self.h, _fp(a) if a is not None else None, w, h, C.byref(st)))
+        return st.as_dict()
+
+    def image_rgb8(self, rgb=None, width=None, height=None) -> np.ndarray:
+        """The 8-bit pixels the reference's save_image writes (main.cpp:26-31),
+        (H, W, 3) in the PNG's row order."""
+        a, w, h = self._framebuffer(rgb, width, height)
+        out = np.empty((h, w, 3), np.uint8)
+        self._chk(N.lib().xnh_image_rgb8(self.h, _fp(a) if a is not None else None, w, h,
+                                         out.ctypes.data_as(N._u8p)))
+        return out
+
+    def save_image(self, path: str, rgb=None, width=None, height=None, report: bool = True) -> dict:
+        """save_image (main.cpp:25-52): the PNG of the running average (or of
+        the given swapchain-layout framebuffer) with the reference's pixel
+        bytes and row order, and its NoiseStats, printed as the reference
+        prints them when `report`."""
+        from .imageio import write_png_rows
+        write_png_rows(path, self.image_rgb8(rgb, width, height))
+        st = self.noise_stats(rgb, width, height)
+        if report:
+            for k in ("mean_squared_score", "rms_score", "mean_absolute", "median_absolute", "p95_absolute",
+                      "max_absolute", "avg_luminance", "edge_threshold", "epsilon", "valid_pixel_count",
+                      "valid_pixel_ratio"):
+                name = "".join(w.capitalize() if i else w for i, w in enumerate(k.split("_")))
+                v = st[k]
+                print(f"{name:<16} = {v if isinstance(v, int) else float(f'{v:.10g}')}")
+            print(f"Saved image to {path}")
+        return st
 
     def read_accum(self):
         out = np.empty((self.height * self.width, 3), np.float32)
