@@ -385,6 +385,28 @@ int xnh_scene_add_primitive(xnh_scene* s, int32_t type, const float* params, int
 int xnh_scene_add_instance(xnh_scene* s, int32_t mesh, const float* t3, float scale, const float* rot_deg3,
                            int32_t instance_id);
 int xnh_scene_build(xnh_scene* s, int32_t nthreads);
+/* The same scene with every mesh's BLAS built on ctx's device by
+ * xnh_build_blas (a linear BVH, not the reference's binned-SAH tree) and
+ * the TLAS and primitives' BVH on the host as xnh_scene_build makes them. */
+int xnh_scene_build_device(xnh_scene* s, xnh_ctx* ctx);
+
+/* ---- BLAS on the device (no counterpart in the reference, which builds
+ * on the CPU: blas.cpp:97-203) ------------------------------------------
+ * A linear BVH (Morton order of the triangle box centres, Karras's radix
+ * tree, boxes bottom-up) in the reference's node format: 2n-1 nodes, one
+ * triangle per leaf, children adjacent and after their parent. The tree
+ * differs from BLAS::build's, so hits agree with the reference's except
+ * where two triangles tie at equal t (shared edges), and images agree
+ * statistically. mesh: vertices and indices (its nodes are ignored);
+ * nodes_out: 2n-1 nodes, prims_out: n slots, *num_nodes_out: 2n-1 (0 for
+ * an empty mesh). */
+int xnh_build_blas(xnh_ctx* ctx, const xnh_mesh* mesh, xnh_bvh_node* nodes_out, uint32_t* prims_out,
+                   int32_t* num_nodes_out);
+/* Recomputes the boxes of mesh's BLAS (any tree in the node format, e.g.
+ * BLAS::build's or xnh_build_blas's) for its current vertices, keeping the
+ * topology: leaves bound their triangles, interior nodes their children.
+ * nodes_out: mesh->num_nodes nodes. */
+int xnh_refit_blas(xnh_ctx* ctx, const xnh_mesh* mesh, xnh_bvh_node* nodes_out);
 /* View of the built scene; valid until the scene is modified or freed. */
 int xnh_scene_flat(const xnh_scene* s, xnh_flat_scene* out);
 /* Camera::look_at (camera/camera.h:15-32). */

@@ -31,7 +31,21 @@ def test_library_exports_every_declared_symbol():
         assert s in N.SIGNATURES, f"{s} has no ctypes signature"
 
 
+def test_library_resolves_its_own_symbols():
+    """Every function the library calls inside its own namespace is linked
+    in (a shared library links with undefined symbols, which would only
+    fail at load or first call on the GPU box)."""
+    import shutil
+    import subprocess
+    if shutil.which("nm") is None:
+        pytest.skip("nm is not installed")
+    out = subprocess.run(["nm", "-D", "--undefined-only", "-C", N.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    assert not [l for l in out.splitlines() if "xnh" in l], out
+
+
 def test_struct_layouts():
+    assert C.sizeof(N.NoiseStats) == 48
     assert C.sizeof(N.BvhNode) == 32          # == BLASNode / TLASNode
     assert C.sizeof(N.Hit) == 64
     assert C.sizeof(N.Ray) == 32

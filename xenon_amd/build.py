@@ -26,7 +26,7 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contra
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-D__HIP_PLATFORM_AMD__",
              f"-I{ROCM}/include"]
 
-DEVICE_SOURCES = ["device/kernels.hip", "device/guide.hip", "device/image.hip"]
+DEVICE_SOURCES = ["device/kernels.hip", "device/guide.hip", "device/image.hip", "device/bvh.hip"]
 HOST_SOURCES = ["host/scene.cpp", "host/scene_api.cpp", "host/scene_io.cpp", "host/api.cpp"]
 
 

@@ -4,6 +4,7 @@
 
 #include "../common/dev_types.h"
 #include "../common/guide_net.h"
+#include "xenon_hip.h"
 
 namespace xnh {
 
@@ -59,5 +60,17 @@ hipError_t launch_to_half(hipStream_t s, const float* src, uint16_t* dst, size_t
 hipError_t launch_img_from_accum(hipStream_t s, const float4* accum, float inv_spp, int w, int h, float* img);
 hipError_t launch_noise_stats(hipStream_t s, const float* img, int w, int h, ImgScratch sc);
 hipError_t launch_img_rgb8(hipStream_t s, const float* img, int w, int h, uint8_t* out);
+
+// BLAS on the device (bvh.hip). build: n >= 2 triangles (vertex indices
+// checked by the caller) -> 2n-1 nodes in the reference's node format and n
+// leaf slots; refit: boxes of an existing tree in that format (structure
+// checked by the caller). parent and flag: nn words of scratch each; err
+// gets a nonzero bit mask if the tree came out malformed. build frees its
+// own scratch and synchronises the stream.
+hipError_t build_blas_device(hipStream_t s, const float* vx, const float* vy, const float* vz, const int32_t* idx,
+                             int n, xnh_bvh_node* nodes, uint32_t* prims, uint32_t* err);
+hipError_t refit_blas_device(hipStream_t s, xnh_bvh_node* nodes, int nn, const uint32_t* prims, int nprims,
+                             const float* vx, const float* vy, const float* vz, const int32_t* idx, uint32_t* parent,
+                             uint32_t* flag, uint32_t* err);
 
 }  // namespace xnh

@@ -1046,6 +1046,143 @@ int xnh_image_rgb8(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, uint8_t* 
   return XNH_OK;
 }
 
+namespace {
+
+// Vertex indices in range (the device kernels index with them unchecked).
+int check_mesh_indices(xnh_ctx* c, const xnh_mesh* m) {
+  const int n = m->num_triangles, nv = m->num_vertices;
+  if (n < 0 || nv < 0 || (n > 0 && (!m->vx || !m->vy || !m->vz || !m->indices)))
+    return fail(c, XNH_E_INVALID, "mesh arrays missing");
+  if (n > (1 << 28)) return fail(c, XNH_E_INVALID, "more than 2^28 triangles");
+  for (int64_t k = 0; k < 3 * static_cast<int64_t>(n); ++k)
+    if (m->indices[k] < 0 || m->indices[k] >= nv) return fail(c, XNH_E_INVALID, "vertex index out of range");
+  return XNH_OK;
+}
+
+struct DevMesh {   // one mesh's vertices and indices on the device, freed on scope exit
+  std::vector<void*> allocs;
+  float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  int32_t* idx = nullptr;
+  ~DevMesh() { free_all(allocs); }
+};
+
+int upload_mesh(xnh_ctx* c, const xnh_mesh* m, DevMesh& d) {
+  const size_t nv = static_cast<size_t>(m->num_vertices), ni = 3 * static_cast<size_t>(m->num_triangles);
+  int rc = dev_alloc(c, d.allocs, &d.vx, nv);
+  if (rc == XNH_OK) rc = dev_alloc(c, d.allocs, &d.vy, nv);
+  if (rc == XNH_OK) rc = dev_alloc(c, d.allocs, &d.vz, nv);
+  if (rc == XNH_OK) rc = dev_alloc(c, d.allocs, &d.idx, ni);
+  if (rc != XNH_OK) return rc;
+  HIP_OR_FAIL(c, hipMemcpyAsync(d.vx, m->vx, sizeof(float) * nv, hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(d.vy, m->vy, sizeof(float) * nv, hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(d.vz, m->vz, sizeof(float) * nv, hipMemcpyHostToDevice, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(d.idx, m->indices, sizeof(int32_t) * ni, hipMemcpyHostToDevice, c->stream));
+  return XNH_OK;
+}
+
+}  // namespace
+
+int xnh_build_blas(xnh_ctx* c, const xnh_mesh* m, xnh_bvh_node* nodes_out, uint32_t* prims_out,
+                   int32_t* num_nodes_out) {
+  if (!c || !m || !num_nodes_out) return XNH_E_INVALID;
+  int rc = check_mesh_indices(c, m);
+  if (rc != XNH_OK) return rc;
+  const int n = m->num_triangles;
+  *num_nodes_out = n > 0 ? 2 * n - 1 : 0;
+  if (n == 0) return XNH_OK;
+  if (!nodes_out || !prims_out) return XNH_E_INVALID;
+  if (n == 1) {   // a single leaf: its box is the triangle's
+    xnh_bvh_node v{};
+    for (int k = 0; k < 3; ++k) {
+      const float* a = k == 0 ? m->vx : (k == 1 ? m->vy : m->vz);
+      const float p0 = a[m->indices[0]], p1 = a[m->indices[1]], p2 = a[m->indices[2]];
+      v.lo[k] = std::fmin(std::fmin(p0, p1), p2);
+      v.hi[k] = std::fmax(std::fmax(p0, p1), p2);
+    }
+    v.child = 0;
+    v.count = 1;
+    nodes_out[0] = v;
+    prims_out[0] = 0;
+    return XNH_OK;
+  }
+  DevMesh d;
+  rc = upload_mesh(c, m, d);
+  if (rc != XNH_OK) return rc;
+  std::vector<void*> own;
+  xnh_bvh_node* nodes = nullptr;
+  uint32_t *prims = nullptr, *err = nullptr;
+  rc = dev_alloc(c, own, &nodes, static_cast<size_t>(2 * n - 1));
+  if (rc == XNH_OK) rc = dev_alloc(c, own, &prims, static_cast<size_t>(n));
+  if (rc == XNH_OK) rc = dev_alloc(c, own, &err, 1);
+  uint32_t herr = 0;
+  hipError_t e = hipSuccess;
+  if (rc == XNH_OK) {
+    e = hipMemsetAsync(err, 0, 4, c->stream);
+    if (e == hipSuccess) e = build_blas_device(c->stream, d.vx, d.vy, d.vz, d.idx, n, nodes, prims, err);
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(nodes_out, nodes, sizeof(xnh_bvh_node) * (2 * n - 1), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(prims_out, prims, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, c->stream);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = es;
+  }
+  free_all(own);
+  if (rc != XNH_OK) return rc;
+  if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("device BLAS build: ") + hipGetErrorString(e));
+  if (herr != 0) return fail(c, XNH_E_HIP, "device BLAS build produced a malformed tree (mask " + std::to_string(herr) + ")");
+  return XNH_OK;
+}
+
+int xnh_refit_blas(xnh_ctx* c, const xnh_mesh* m, xnh_bvh_node* nodes_out) {
+  if (!c || !m || !nodes_out) return XNH_E_INVALID;
+  int rc = check_mesh_indices(c, m);
+  if (rc != XNH_OK) return rc;
+  const int nn = m->num_nodes, n = m->num_triangles;
+  if (nn <= 0) return XNH_OK;
+  if (!m->nodes || !m->prims) return fail(c, XNH_E_INVALID, "mesh has no BLAS");
+  // the kernels trust the structure: children after their parent, leaves
+  // inside the slot array, slots naming triangles
+  for (int j = 0; j < nn; ++j) {
+    const xnh_bvh_node& v = m->nodes[j];
+    if (v.count == 0 && (v.child <= static_cast<uint32_t>(j) || static_cast<int64_t>(v.child) + 1 >= nn))
+      return fail(c, XNH_E_INVALID, "BLAS child outside node range");
+    if (v.count > 0 && static_cast<int64_t>(v.child) + v.count > n)
+      return fail(c, XNH_E_INVALID, "BLAS leaf outside slot range");
+  }
+  for (int k = 0; k < n; ++k)
+    if (m->prims[k] >= static_cast<uint32_t>(n)) return fail(c, XNH_E_INVALID, "slot names no triangle");
+  DevMesh d;
+  rc = upload_mesh(c, m, d);
+  if (rc != XNH_OK) return rc;
+  std::vector<void*> own;
+  xnh_bvh_node* nodes = nullptr;
+  uint32_t *prims = nullptr, *parent = nullptr, *flag = nullptr, *err = nullptr;
+  rc = dev_alloc(c, own, &nodes, static_cast<size_t>(nn));
+  if (rc == XNH_OK) rc = dev_alloc(c, own, &prims, static_cast<size_t>(n));
+  if (rc == XNH_OK) rc = dev_alloc(c, own, &parent, static_cast<size_t>(nn));
+  if (rc == XNH_OK) rc = dev_alloc(c, own, &flag, static_cast<size_t>(nn));
+  if (rc == XNH_OK) rc = dev_alloc(c, own, &err, 1);
+  uint32_t herr = 0;
+  hipError_t e = hipSuccess;
+  if (rc == XNH_OK) {
+    e = hipMemcpyAsync(nodes, m->nodes, sizeof(xnh_bvh_node) * nn, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(prims, m->prims, sizeof(uint32_t) * n, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(err, 0, 4, c->stream);
+    if (e == hipSuccess) e = refit_blas_device(c->stream, nodes, nn, prims, n, d.vx, d.vy, d.vz, d.idx, parent, flag, err);
+    if (e == hipSuccess) e = hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(nodes_out, nodes, sizeof(xnh_bvh_node) * nn, hipMemcpyDeviceToHost, c->stream);
+    const hipError_t es = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) e = es;
+  }
+  free_all(own);
+  if (rc != XNH_OK) return rc;
+  if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("device BLAS refit: ") + hipGetErrorString(e));
+  if (herr != 0) return fail(c, XNH_E_HIP, "device BLAS refit found a malformed tree (mask " + std::to_string(herr) + ")");
+  return XNH_OK;
+}
+
 int xnh_read_accum(xnh_ctx* c, float* sum_out, int32_t* spp_out) {
   if (!c) return XNH_E_INVALID;
   const int npix = c->cfg.width * c->cfg.height;

@@ -139,13 +139,21 @@ int xnh_scene_add_instance(xnh_scene* s, int32_t mesh, const float* t, float sca
   return static_cast<int>(s->s.pending.size()) - 1;
 }
 
-int xnh_scene_build(xnh_scene* s, int32_t nthreads) {
-  (void)nthreads;
-  if (!s) return XNH_E_INVALID;
-  HostScene& hs = s->s;
-  for (const auto& p : hs.pending)
-    if (p.mesh >= static_cast<int32_t>(hs.meshes.size())) return XNH_E_INVALID;
-  for (auto& m : hs.meshes) build_blas(m);
+namespace {
+
+xnh_mesh geometry_view(const HostMesh& m) {
+  xnh_mesh v{};
+  v.num_vertices = static_cast<int32_t>(m.vx.size());
+  v.num_triangles = m.tris();
+  v.vx = m.vx.data();
+  v.vy = m.vy.data();
+  v.vz = m.vz.data();
+  v.indices = m.indices.data();
+  return v;
+}
+
+// The TLAS, the primitives' BVH and the flat views, once every BLAS exists.
+int finish_build(HostScene& hs) {
   build_prims(hs);
   build_tlas(hs);
   hs.mesh_views.resize(hs.meshes.size());
@@ -168,6 +176,35 @@ int xnh_scene_build(xnh_scene* s, int32_t nthreads) {
   }
   hs.built = true;
   return XNH_OK;
+}
+
+}  // namespace
+
+int xnh_scene_build(xnh_scene* s, int32_t nthreads) {
+  (void)nthreads;
+  if (!s) return XNH_E_INVALID;
+  HostScene& hs = s->s;
+  for (const auto& p : hs.pending)
+    if (p.mesh >= static_cast<int32_t>(hs.meshes.size())) return XNH_E_INVALID;
+  for (auto& m : hs.meshes) build_blas(m);
+  return finish_build(hs);
+}
+
+int xnh_scene_build_device(xnh_scene* s, xnh_ctx* ctx) {
+  if (!s || !ctx) return XNH_E_INVALID;
+  HostScene& hs = s->s;
+  for (const auto& p : hs.pending)
+    if (p.mesh >= static_cast<int32_t>(hs.meshes.size())) return XNH_E_INVALID;
+  for (auto& m : hs.meshes) {
+    const xnh_mesh v = geometry_view(m);
+    const size_t n = static_cast<size_t>(m.tris());
+    m.nodes.assign(n > 0 ? 2 * n - 1 : 0, xnh_bvh_node{});
+    m.prims.assign(n, 0u);
+    int32_t nn = 0;
+    const int rc = xnh_build_blas(ctx, &v, m.nodes.data(), m.prims.data(), &nn);
+    if (rc != XNH_OK) return rc;
+  }
+  return finish_build(hs);
 }
 
 int xnh_scene_flat(const xnh_scene* s, xnh_flat_scene* out) {

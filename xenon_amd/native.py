@@ -182,6 +182,9 @@ SIGNATURES = {
     "xnh_scene_add_instance": (C.c_int, [_vp, C.c_int32, _fp, C.c_float, _fp, C.c_int32]),
     "xnh_scene_add_primitive": (C.c_int, [_vp, C.c_int32, _fp, C.c_int32]),
     "xnh_scene_build": (C.c_int, [_vp, C.c_int32]),
+    "xnh_scene_build_device": (C.c_int, [_vp, _vp]),
+    "xnh_build_blas": (C.c_int, [_vp, C.POINTER(MeshView), C.POINTER(BvhNode), _u32p, _ip]),
+    "xnh_refit_blas": (C.c_int, [_vp, C.POINTER(MeshView), C.POINTER(BvhNode)]),
     "xnh_scene_flat": (C.c_int, [_vp, C.POINTER(FlatScene)]),
     "xnh_camera_look_at": (C.c_int, [_fp, _fp, _fp, C.c_float, C.c_float, C.POINTER(Camera)]),
     "xnh_scene_load_file": (C.c_int, [C.c_char_p, C.POINTER(_vp), C.POINTER(Camera), C.POINTER(FileInfo)]),

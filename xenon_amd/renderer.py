@@ -34,7 +34,11 @@ def _ip(a):
 class HipScene:
     """Scene built by the backend's host builder from a ``SceneDesc``."""
 
-    def __init__(self, desc: SceneDesc):
+    def __init__(self, desc: SceneDesc, device_blas: "HipRenderer | None" = None):
+        """desc's scene, built on the host as Scene::build_acceleration builds
+        it (the reference's trees); with device_blas, every mesh's BLAS is
+        built on that renderer's device instead (xnh_build_blas: a linear
+        BVH, hits equal up to equal-t ties)."""
         L = N.lib()
         self.desc = desc
         self.h = L.xnh_scene_new()
@@ -67,7 +71,10 @@ class HipScene:
             t = np.array(ins.translation, np.float32)
             r = np.array(ins.rot_deg, np.float32)
             N.check(L.xnh_scene_add_instance(self.h, ins.mesh, _fp(t), ins.scale, _fp(r), ins.instance_id))
-        N.check(L.xnh_scene_build(self.h, 0))
+        if device_blas is None:
+            N.check(L.xnh_scene_build(self.h, 0))
+        else:
+            N.check(L.xnh_scene_build_device(self.h, device_blas.h), device_blas.h)
         self._flat = N.FlatScene()
         N.check(L.xnh_scene_flat(self.h, C.byref(self._flat)))
 
@@ -266,6 +273,47 @@ class HipRenderer:
         out = np.empty((self.height * self.width, 3), np.float32) if want_image else None
         self._chk(N.lib().xnh_render(self.h, C.byref(camera), int(spp_first), int(spp_count),
                                      _fp(out) if out is not None else None))
+        return out
+
+    # -- BLAS on the device ------------------------------------------------
+    @staticmethod
+    def _mesh_view(positions, indices, nodes=None, prims=None):
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+        idx = np.ascontiguousarray(indices, np.int32).reshape(-1, 3)
+        v = N.MeshView()
+        v.num_vertices, v.num_triangles = pos.shape[0], idx.shape[0]
+        v.vx, v.vy, v.vz, v.indices = _fp(x), _fp(y), _fp(z), _ip(idx)
+        keep = [x, y, z, idx]
+        if nodes is not None:
+            nd = np.ascontiguousarray(nodes)
+            pr = np.ascontiguousarray(prims, np.uint32)
+            v.num_nodes = nd.shape[0]
+            v.nodes = nd.ctypes.data_as(C.POINTER(N.BvhNode))
+            v.prims = pr.ctypes.data_as(N._u32p)
+            keep += [nd, pr]
+        return v, keep
+
+    def build_blas(self, positions, indices):
+        """A BLAS built on the device (xnh_build_blas): (nodes, leaf slot ->
+        triangle), nodes in NODE_DTYPE."""
+        v, keep = self._mesh_view(positions, indices)
+        n = v.num_triangles
+        nodes = np.zeros(max(2 * n - 1, 0), NODE_DTYPE)
+        prims = np.zeros(n, np.uint32)
+        nn = C.c_int32()
+        self._chk(N.lib().xnh_build_blas(self.h, C.byref(v), nodes.ctypes.data_as(C.POINTER(N.BvhNode)),
+                                         prims.ctypes.data_as(N._u32p), C.byref(nn)))
+        del keep
+        return nodes[:nn.value], prims
+
+    def refit_blas(self, positions, indices, nodes, prims):
+        """The boxes of (nodes, prims) recomputed for new vertex positions
+        (xnh_refit_blas); the topology is kept."""
+        v, keep = self._mesh_view(positions, indices, np.asarray(nodes, NODE_DTYPE), prims)
+        out = np.zeros(v.num_nodes, NODE_DTYPE)
+        self._chk(N.lib().xnh_refit_blas(self.h, C.byref(v), out.ctypes.data_as(C.POINTER(N.BvhNode))))
+        del keep
         return out
 
     # -- output stage (main.cpp:25-52, math/image.h) ---------------------
