@@ -174,8 +174,29 @@ def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, fre
             "median_ratio_plain_over_guided": ratio("median")}
 
 
+class _StdoutToStderr:
+    """Sends what native code writes to stdout (the reference's build prints
+    progress lines) to stderr, so that stdout holds only the JSON line."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
 def cpu_baseline(desc, budget_s: float):
     """Times the CPU integrator on this host; returns the cpu_baseline object."""
+    with _StdoutToStderr():
+        return _cpu_baseline(desc, budget_s)
+
+
+def _cpu_baseline(desc, budget_s: float):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
 
