@@ -430,7 +430,10 @@ int xnh_camera_look_at(const float* eye3, const float* target3, const float* up3
  * places mesh number <mesh> (file order) with the TRS of
  * xnh_scene_add_instance; id defaults to the instance's index. With any
  * instance line the TLAS holds exactly those instances; without, one
- * identity instance per mesh, as load_scene builds. A bad instance line
+ * identity instance per mesh, as load_scene builds. Either way the
+ * primitives, if any, follow as one identity instance (mesh
+ * XNH_PRIM_GEOMETRY, id = the number of instances before it), as
+ * Scene::build_acceleration adds them (scene/scene.h:97-110). A bad instance line
  * (unknown mesh, missing fields) fails the load with XNH_E_INVALID. */
 typedef struct {
   int32_t width, height, samples;       /* SceneConfig (scene/scene_file.h:11-19) */
