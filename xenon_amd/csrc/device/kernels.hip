@@ -510,11 +510,14 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_accumulate(WaveBuffers B) {
   __shared__ BatchView v;
+  __shared__ unsigned long long red[2][256 / 64];
   load_batch(v, B, nullptr, true);
-  const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long bad = 0;
   unsigned long long paths = 0;
-  if (gid < v.prefix[v.nch]) {
+  // grid-stride over the batch's pixels (the grid is sized for the largest
+  // batch, most of whose entries have no pixel)
+  const uint32_t total = v.prefix[v.nch];
+  for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += gridDim.x * blockDim.x) {
     const int ci = find_chunk(v, gid);
     const DevChunk& ck = v.ch[ci];
     const int p = static_cast<int>(gid - v.prefix[ci]);
@@ -530,12 +533,24 @@ __global__ void __launch_bounds__(256) k_accumulate(WaveBuffers B) {
       a.z = a.z + L.z;
     }
     accum[p] = a;
-    paths = static_cast<unsigned long long>(ck.nsamples);
+    paths += static_cast<unsigned long long>(ck.nsamples);
   }
-  const unsigned long long tot = wave_sum(bad), np = wave_sum(paths);
+  // one atomic per block and counter
+  const unsigned long long tb = wave_sum(bad), tp = wave_sum(paths);
+  const int w = static_cast<int>(threadIdx.x >> 6);
   if (lane_id() == 0) {
-    if (tot) atomicAdd(&B.stats[kStNonFinite], tot);
-    if (np) atomicAdd(&B.stats[kStPaths], np);
+    red[0][w] = tb;
+    red[1][w] = tp;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long sb = 0, sp = 0;
+    for (int k = 0; k < static_cast<int>(blockDim.x >> 6); ++k) {
+      sb += red[0][k];
+      sp += red[1][k];
+    }
+    if (sb) atomicAdd(&B.stats[kStNonFinite], sb);
+    if (sp) atomicAdd(&B.stats[kStPaths], sp);
   }
 }
 
@@ -689,7 +704,8 @@ hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int gr
 
 hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int max_pixels) {
   const int n = max_pixels > 1 ? max_pixels : 1;
-  hipLaunchKernelGGL(k_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, B);
+  const int blocks = (n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048;
+  hipLaunchKernelGGL(k_accumulate, dim3(blocks), dim3(256), 0, s, B);
   return hipGetLastError();
 }
 
