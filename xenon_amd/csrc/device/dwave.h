@@ -38,6 +38,29 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 
 // Wave-reduced ray and (with `work`) traversal counters; `any` selects the
 // shadow slots.
+// The persistent traversal kernels' counts, summed over the work-group and
+// added with one atomic per counter and work-group: the waves of a launch
+// finish together, and one same-address atomic per wave at the end made a
+// serial tail. Every thread of the work-group must reach it (256 threads).
+__device__ __forceinline__ void flush_counts_block(unsigned long long* stats, const TravCount& c, uint32_t rays,
+                                                   bool any, bool work) {
+  __shared__ unsigned long long red[5][256 / 64];
+  const int w = static_cast<int>(threadIdx.x >> 6), nw = static_cast<int>(blockDim.x >> 6);
+  const unsigned long long v[5] = {wave_sum(rays), work ? wave_sum(c.node_tests) : 0ull,
+                                   work ? wave_sum(c.interior) : 0ull, work ? wave_sum(c.tri_tests) : 0ull,
+                                   work ? wave_sum(c.inst_visits) : 0ull};
+  if (lane_id() == 0)
+    for (int k = 0; k < 5; ++k) red[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 5u) {
+    unsigned long long t = 0;
+    for (int k = 0; k < nw; ++k) t += red[threadIdx.x][k];
+    const int base = any ? kStShNodeTests : kStNodeTests;
+    const int slot = threadIdx.x == 0 ? (any ? kStShadow : kStClosest) : base + static_cast<int>(threadIdx.x) - 1;
+    if (t) atomicAdd(&stats[slot], t);
+  }
+}
+
 __device__ __forceinline__ void flush_counts(unsigned long long* stats, const TravCount& c, uint32_t rays, bool any,
                                              bool work = true) {
   const unsigned long long r = wave_sum(rays);

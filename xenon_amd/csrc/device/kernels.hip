@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
         B.hit[i] = make_float4(w.found ? w.tbest : 0.f, w.hit.u, w.hit.v, __int_as_float(w.found ? w.hit.inst : -1));
         B.hit_slot[i] = w.hit.slot;
       });
-  flush_counts(B.stats, c, rays, false, kCount);
+  flush_counts_block(B.stats, c, rays, false, kCount);
   if (overflow) atomicAdd(&B.counters[kOverflow], 1u);
 }
 
@@ -499,7 +499,7 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
           B.l_nee[k] = acc;
         }
       });
-  flush_counts(B.stats, c, rays, true, kCount);
+  flush_counts_block(B.stats, c, rays, true, kCount);
   if (overflow) atomicAdd(&B.counters[kOverflow], 1u);
 }
 
