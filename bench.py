@@ -113,8 +113,12 @@ def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, fre
     difference of two halves has four times the variance of the full
     reference), and that estimate is subtracted. The guide uses the
     library's defaults (16-level hash grid of 2^19 entries x 2 FP16
-    features, 3x64 MLP, 8 vMF lobes, training on up to 2^18 samples after
-    every batch); the guided run's time includes its training."""
+    features, 3x64 MLP, 8 vMF lobes; after every batch one training step
+    per 2^23 of its paths, on slices of up to 2^18 samples drawn from it).
+    The plain run renders in the library's default batches (about 2^27
+    paths), the guided run in whole guided batches (the library caps them at
+    2^26 paths, so that the guide learns often); the guided run's time
+    includes its training."""
     import numpy as np
     from xenon_amd import HipRenderer, HipScene, scenes
 
@@ -141,19 +145,22 @@ def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, fre
     log(f"guided: reference {ref_spp} spp in {time.perf_counter() - t0:.1f} s")
 
     def run(guided: bool):
-        r.reset()
+        rr = r
+        rr.reset()
         if guided:
-            r.guide_enable()
-        chunk = max(r.tuning()["spp_per_launch"], 1)
+            rr.guide_enable()
+        chunk = max(rr.tuning()["spp_per_launch"], 1)
+        if guided:   # whole guided batches (the library caps them at 2^26 paths)
+            chunk = max(1, min(chunk, (1 << 26) // (desc.width * desc.height)))
         t = time.perf_counter()
         spp = 0
         while time.perf_counter() - t < seconds:
-            r.render(cam, 1 + spp, chunk, want_image=False)
+            rr.render(cam, 1 + spp, chunk, want_image=False)
             spp += chunk
         el = time.perf_counter() - t
-        img, n = r.read_accum()
+        img, n = rr.read_accum()
         if guided:
-            r.guide_disable()
+            rr.guide_disable()
         return img / n, n, el
 
     plain, sp, tp = run(False)
@@ -162,8 +169,11 @@ def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, fre
     log(f"guided: plain {sp} spp {ep}; guided {sg} spp {eg}")
     r.close()
 
+    # runs stop at the first batch boundary past the budget, so their
+    # wall-clock differs a little; relMSE x seconds (the usual efficiency)
+    # corrects for that, since relMSE falls as 1/samples
     def ratio(k):
-        return round(ep[k] / eg[k], 3) if eg[k] > 0 else None
+        return round(ep[k] * tp / (eg[k] * tg), 3) if eg[k] > 0 else None
 
     return {"workload": f"caustic room ({desc.num_tris} tris), {desc.width}x{desc.height}, depth {desc.max_bounces}",
             "seconds": round(seconds, 2), "reference_spp": ref_spp,

@@ -259,7 +259,9 @@ typedef struct {
   float bounds_lo[3], bounds_hi[3];   /* scene box mapped to the unit cube */
   int32_t train_batch;     /* training samples per batch (0: 2^18) */
   int32_t record_paths;    /* paths per batch that record training samples (0: all) */
-  int32_t train_steps;     /* optimisation steps per batch, each on an equal slice of its records (0: 1) */
+  int32_t train_steps;     /* optimisation steps per batch, each on an equal slice of its records
+                              (0: one per 2^23 paths of a guided batch, 1 to 64; guided batches hold at
+                              most 2^26 paths, or one job's pixels if more) */
   int32_t warmup_steps;    /* the guide's MIS probability ramps from 0 to guide_fraction over this many steps */
   int32_t guide_depth;     /* guide the bounces below this depth (0: 1, the camera vertex only: the best
                               equal-time trade-off measured on the caustic room); deeper vertices sample the BSDF */
@@ -297,8 +299,8 @@ int xnh_guide_train(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm,
  * 256-thread blocks at cuda_renderer.cu:29) ----
  * Workgroup size of the traversal kernels (64..256, multiple of 64), LDS
  * traversal-stack entries per lane (1..128; deeper entries spill to HBM) and
- * samples per pixel per batch (sets the queue capacity; 0 = auto, about 8M
- * paths). Changing them reallocates the queues and clears the running
+ * samples per pixel per batch (sets the queue capacity; 0 = auto, about
+ * 2^27 paths, at most 256 samples: 292 B per path, about 37 GB). Changing them reallocates the queues and clears the running
  * accumulation. */
 typedef struct {
   int32_t block_size, lds_stack, spp_per_launch;
