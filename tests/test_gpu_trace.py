@@ -105,3 +105,30 @@ def test_empty_batch_and_errors(pair):
     assert hits.shape == (0,)
     occ = r.trace_any(np.zeros((0, 8), np.float32))
     assert occ.shape == (0,)
+
+
+def test_large_leaves(oracle):
+    """Leaves of 15 or more triangles (repeated triangles the builder cannot
+    split) take the large-leaf stack entry, whose count is looked up per
+    slot: hits and occlusion stay bit-exact."""
+    from xenon_amd import HipRenderer, HipScene, scenes
+    desc = scenes.cornell(32, 32)
+    P = np.array([[1.0, 1.0, -2.0], [4.0, 1.2, -2.5], [2.0, 4.0, -2.2], [1.5, 0.5, -1.0], [3.5, 0.8, -1.2]],
+                 np.float32)
+    F = np.concatenate([np.tile([[0, 1, 2]], (40, 1)), np.tile([[0, 3, 4]], (17, 1))]).astype(np.int32)
+    desc.meshes.append(scenes.Mesh(P, F, np.zeros(len(F), np.int32)))
+    hs = HipScene(desc)
+    nodes, _ = hs.blas(1)
+    assert nodes["count"].max() >= 15, "the test needs a large leaf"
+    osc = oracle.load(desc)
+    rays = standard_batch(desc, osc.camera(), 2048, seed=9)
+    hf, hi = osc.trace_closest(rays)
+    with HipRenderer(32, 32, 8, 3) as r:
+        r.upload_scene(hs)
+        hits = r.trace_closest(rays)
+        id_bad, f_bad, _ = _compare_closest(hf, hi, hits)
+        print(f"  large leaves: {len(rays)} rays, {int((hi[:, 0] == 1).sum())} on the repeated triangles, "
+              f"mismatches {int((id_bad | f_bad).sum())}", flush=True)
+        assert not np.any(id_bad | f_bad)
+        srays = shadow_rays(rays, hf, hi, np.random.default_rng(10))
+        assert np.array_equal(r.trace_any(srays), osc.trace_any(srays))

@@ -3,9 +3,12 @@
 //
 // Scene (read-only, built once by upload_scene):
 //   bnodes  2 x float4 per BLAS node, all meshes concatenated:
-//           {lo.x, lo.y, lo.z, hi.x} {hi.y, hi.z, child, count}
+//           {lo.x, lo.y, lo.z, hi.x} {hi.y, hi.z, child, entry}
 //           interior: child = global index of the left child (right = +1);
-//           leaf: child = global triangle slot of its first triangle.
+//           leaf: child = global triangle slot of its first triangle;
+//           entry = the node's traversal stack entry (see dtrace.h)
+//   bigcnt  per global triangle slot: the triangle count of a leaf of 15
+//           or more triangles starting there (0 elsewhere)
 //           Each mesh starts at an odd global index, so every child pair
 //           (left index odd within the mesh) is 64-byte aligned.
 //   tris    3 x float4 per leaf slot, in leaf order:
@@ -61,6 +64,7 @@ struct DevSky {
 
 struct DevScene {
   const float4* bnodes;
+  const uint32_t* bigcnt;
   const float4* tris;
   const float4* tnodes;
   const uint32_t* torder;

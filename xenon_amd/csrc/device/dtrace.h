@@ -13,11 +13,12 @@
 //     is shared by both spaces (geometry/transform.h:70-77).
 //
 // BLAS stack entries carry what the walk needs about a node without loading
-// it: the node's child index and triangle count, read from the node record
-// when its parent pair-tests it (blas_entry). Bits 28..31 hold the count
-// (0 = interior: low bits = the left child of the pair; 1..14 = leaf: low
-// bits = its first triangle slot; 15 = large leaf: low bits = the node
-// index, whose record is loaded on pop). Node and slot indices are < 2^28.
+// it: the node's child index and triangle count. The upload precomputes each
+// node's entry into its record, so a pair test reads both children's entries
+// with their boxes. Bits 28..31 hold the count (0 = interior: low bits = the
+// left child of the pair; 1..14 = leaf: low bits = its first triangle slot;
+// 15 = a leaf of 15 or more: low bits = its first slot, whose bigcnt word
+// holds the count). Node and slot indices are < 2^28.
 //
 // Stack: the top `lds_depth` entries of each lane live in LDS, laid out
 // entry-major ([entry][lane]) so lanes at equal depth hit distinct banks;
@@ -146,12 +147,6 @@ __device__ __forceinline__ bool all_finite(f3 v) {
 
 // Written with selects rather than branches: it runs for both children of
 // every interior step.
-__device__ __forceinline__ uint32_t blas_entry(uint32_t node, uint32_t child, uint32_t count) {
-  const uint32_t c = min(count, kEntBigLeaf);
-  const uint32_t leaf = (c << kEntShift) | (count < kEntBigLeaf ? child : node);
-  return count == 0u ? child : leaf;
-}
-
 // Instance: world_to_local columns and metadata (see dev_types.h).
 struct InstRec {
   f3 c0, c1, c2, c3;
@@ -604,8 +599,7 @@ struct Walker {
     }
     const int m = kFast ? pair_test_fast(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1)
                         : pair_test(la, lb, ra, rb, linv, lo, tmin, tbest, tn0, tn1);
-    const uint32_t el = blas_entry(child, __float_as_uint(lb.z), __float_as_uint(lb.w));
-    const uint32_t er = blas_entry(child + 1, __float_as_uint(rb.z), __float_as_uint(rb.w));
+    const uint32_t el = __float_as_uint(lb.w), er = __float_as_uint(rb.w);
     uint32_t next;
     if (m == 3) {
       // closest: nearer first, ties right first (blas.cpp:252-258). Any
@@ -630,12 +624,9 @@ struct Walker {
 
   // Tests the held leaf; true when an any-hit walk found an occluder.
   __device__ __forceinline__ bool leaf_step(const DevScene& S, TravStack& st, TravCount& c) {
-    uint32_t count = leaf >> kEntShift, first = leaf & kEntIndexMask;
-    if (count == kEntBigLeaf) {
-      const float4 nb = gld4(S.bnodes, 2 * first + 1);
-      first = __float_as_uint(nb.z);
-      count = __float_as_uint(nb.w);
-    }
+    uint32_t count = leaf >> kEntShift;
+    const uint32_t first = leaf & kEntIndexMask;
+    if (count == kEntBigLeaf) count = gld1(S.bigcnt, first);
     for (uint32_t k = 0; k < count; ++k) {
       float t, u, v;
       if (kCount) ++c.tri_tests;

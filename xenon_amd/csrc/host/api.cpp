@@ -240,6 +240,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
     pnodes[2 * j + 1] = f4(n.hi[1], n.hi[2], bits(n.child), bits(n.count));
   }
   std::vector<float4> bnodes(2 * nn), tris(3 * ns), vnrm(nv);
+  std::vector<uint32_t> bigcnt(std::max<size_t>(ns, 1), 0u);   // triangles of a large leaf, at its first slot
   std::vector<int32_t> vidx(3 * ns), tmat(ns);
   std::vector<bool> flat(nm, false);
   for (int m = 0; m < nm; ++m) {
@@ -253,8 +254,13 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
       // every walk then moves to strictly larger indices and must end
       if (n.count == 0 && (n.child + 1 >= static_cast<uint32_t>(me.num_nodes) || n.child <= static_cast<uint32_t>(j)))
         return fail(c, XNH_E_INVALID, "BLAS child outside node range");
+      // the node's stack entry, precomputed (see dtrace.h): interior: its
+      // left child; leaf: count << kEntShift | first slot, a large leaf
+      // with the count kept in bigcnt at that slot
+      const uint32_t entry = n.count == 0 ? child : (std::min(n.count, kEntBigLeaf) << kEntShift) | child;
+      if (n.count >= kEntBigLeaf) bigcnt[child] = n.count;
       bnodes[2 * (node_base[m] + j)] = f4(n.lo[0], n.lo[1], n.lo[2], n.hi[0]);
-      bnodes[2 * (node_base[m] + j) + 1] = f4(n.hi[1], n.hi[2], bits(child), bits(n.count));
+      bnodes[2 * (node_base[m] + j) + 1] = f4(n.hi[1], n.hi[2], bits(child), bits(entry));
     }
     for (int s = 0; s < me.num_triangles; ++s) {
       const uint32_t t = me.prims[s];
@@ -336,10 +342,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
     }
     uint32_t root = kNoRoot;
     if (fs->meshes[in.mesh].num_nodes > 0) {
-      const uint32_t rn = node_base[in.mesh];
-      const float4 rb = bnodes[2 * rn + 1];
-      const uint32_t rc = as_uint(rb.z), cnt = as_uint(rb.w);
-      root = cnt == 0 ? rc : (cnt < kEntBigLeaf ? (cnt << kEntShift) | rc : (kEntBigLeaf << kEntShift) | rn);
+      root = as_uint(bnodes[2 * node_base[in.mesh] + 1].w);   // the root node's entry
     }
     inst[4 * i] = f4(w[0], w[1], w[2], w[3]);
     inst[4 * i + 1] = f4(w[4], w[5], w[6], w[7]);
@@ -386,9 +389,10 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   free_all(g->allocs);
   DevScene S{};
   float4 *p_bn, *p_tr, *p_tn, *p_in, *p_vn, *p_li, *p_ma, *p_su, *p_pr = nullptr, *p_pn = nullptr;
-  uint32_t *p_to, *p_po = nullptr;
+  uint32_t *p_to, *p_po = nullptr, *p_bc;
   int32_t *p_vi, *p_tm, *p_lm;
   int rc = dev_upload(c, g->allocs, &p_bn, bnodes);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_bc, bigcnt);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_tr, tris);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_tn, tnodes);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_to, torder);
@@ -408,6 +412,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
     return rc;
   }
   S.bnodes = p_bn;
+  S.bigcnt = p_bc;
   S.tris = p_tr;
   S.tnodes = p_tn;
   S.torder = p_to;
@@ -437,7 +442,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   g->bytes = 0;
   for (const auto* v : {&bnodes, &tris, &tnodes, &inst, &vnrm, &lights, &mats, &suns, &prims, &pnodes})
     g->bytes += v->size() * 16;
-  g->bytes += (torder.size() + vidx.size() + tmat.size() + lmap.size() + porder.size()) * 4;
+  g->bytes += (torder.size() + vidx.size() + tmat.size() + lmap.size() + porder.size() + bigcnt.size()) * 4;
   return XNH_OK;
 }
 
