@@ -300,7 +300,7 @@ int xnh_guide_train(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm,
  * Workgroup size of the traversal kernels (64..256, multiple of 64), LDS
  * traversal-stack entries per lane (1..128; deeper entries spill to HBM) and
  * samples per pixel per batch (sets the queue capacity; 0 = auto, about
- * 2^27 paths, at most 256 samples: 292 B per path, about 37 GB). Changing them reallocates the queues and clears the running
+ * 2^28 paths, at most 256 samples: 292 B per path, about 75 GB). Changing them reallocates the queues and clears the running
  * accumulation. */
 typedef struct {
   int32_t block_size, lds_stack, spp_per_launch;

@@ -813,12 +813,13 @@ int xnh_create(const xnh_config* cfg, xnh_ctx** out) {
   if (cfg->spp_per_launch > 0) {
     c->nsamples = cfg->spp_per_launch;
   } else {
-    // paths in flight per batch: 2^27 (about 37 GB of queues at 292 B per
-    // path). Each launch pays a fixed tail (its last, longest walks) and
-    // every batch the kernel boundaries of all its bounces, so throughput
-    // keeps rising with the batch: 1080p benchmark, 4 samples per batch
-    // (8.3 M paths) 559 Mpaths/s, 16 samples 834, 64 samples 960.
-    const long long target = 1LL << 27;
+    // paths in flight per batch: 2^28 (about 75 GB of queues at 292 B per
+    // path, a quarter of the device). Each launch pays a fixed tail (its
+    // last, longest walks) and every batch the kernel boundaries of all its
+    // bounces, so throughput keeps rising with the batch: 1080p benchmark,
+    // 4 samples per batch (8.3 M paths) 559 Mpaths/s, 16 samples 834,
+    // 64 samples 960 (977 with later changes), 128 samples 1001.
+    const long long target = 1LL << 28;
     c->nsamples = static_cast<int>(std::max(1LL, std::min(256LL, target / npix)));
   }
   if (npix * c->nsamples > (1LL << 30)) {
