@@ -114,7 +114,7 @@ def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, fre
     reference), and that estimate is subtracted. The guide uses the
     library's defaults (16-level hash grid of 2^19 entries x 2 FP16
     features, 3x64 MLP, 8 vMF lobes; after every batch one training step
-    per 2^23 of its paths, on slices of up to 2^18 samples drawn from it).
+    per 2^22 of its paths, on slices of up to 2^18 samples drawn from it).
     The plain run renders in the library's default batches (about 2^27
     paths), the guided run in whole guided batches (the library caps them at
     2^26 paths, so that the guide learns often); the guided run's time

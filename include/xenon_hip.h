@@ -260,7 +260,7 @@ typedef struct {
   int32_t train_batch;     /* training samples per batch (0: 2^18) */
   int32_t record_paths;    /* paths per batch that record training samples (0: all) */
   int32_t train_steps;     /* optimisation steps per batch, each on an equal slice of its records
-                              (0: one per 2^23 paths of a guided batch, 1 to 64; guided batches hold at
+                              (0: one per 2^22 paths of a guided batch, 1 to 64; guided batches hold at
                               most 2^26 paths, or one job's pixels if more) */
   int32_t warmup_steps;    /* the guide's MIS probability ramps from 0 to guide_fraction over this many steps */
   int32_t guide_depth;     /* guide the bounces below this depth (0: 1, the camera vertex only: the best

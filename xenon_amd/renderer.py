@@ -467,10 +467,11 @@ class HipRenderer:
 
     def guide_enable(self, bounds_lo=None, bounds_hi=None, t_log2: int = 19, learning_rate: float = 5e-3,
                      guide_fraction: float = 0.5, seed: int = 0, train_batch: int = 0, record_paths: int = 0,
-                     train_steps: int = 1, warmup_steps: int = 0, guide_depth: int = 0):
+                     train_steps: int = 0, warmup_steps: int = 0, guide_depth: int = 0):
         """Turns learned path guiding on with fresh parameters; positions
         inside [bounds_lo, bounds_hi] (default: the uploaded scene's box)
-        are mapped to the hash grid."""
+        are mapped to the hash grid. Zero arguments take the library's
+        defaults (xnh_guide_config)."""
         if bounds_lo is None or bounds_hi is None:
             if self.scene is None:
                 raise ValueError("guide_enable needs bounds or an uploaded scene")
