@@ -1370,9 +1370,9 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (cfg.record_paths <= 0 || cfg.record_paths > c->B.capacity) cfg.record_paths = c->B.capacity;
   // one optimisation step per 2^22 paths a guided batch holds: the guide
   // then learns at the same rate per rendered sample whatever the batch
-  // size. Caustic room, 2^26-path batches, equal time against plain: 8
-  // steps per batch 1.06x (trimmed relMSE) / 1.08x (median pixel error),
-  // 16 steps 1.13x / 1.36x, 32 steps 1.14x / 1.37x.
+  // size. Caustic room, 2^26-path batches, equal time against plain: 1
+  // step per batch 1.06x (trimmed relMSE) / 1.08x (median pixel error),
+  // 8 steps 1.12x / 1.33x, 16 steps 1.13x / 1.36x, 32 steps 1.14x / 1.37x.
   if (cfg.train_steps <= 0)
     cfg.train_steps = std::max(
         1, std::min(64, static_cast<int>(std::min<size_t>(static_cast<size_t>(c->B.capacity), kGuidedBatchPaths) >> 22)));
