@@ -7,8 +7,9 @@ parameters, so its image includes the untrained start. Errors are per-pixel
 relative squared errors against a plain render of 16x the samples from
 disjoint sample indices. Guiding must lower the median pixel error by at
 least 5 % (measured 1.11-1.12x over four runs) and must not raise the mean
-without the 0.1 % largest pixel errors (1.01-1.23x lower over the same runs;
-training order on the device varies, so that measure moves most). The full
+without the 0.1 % largest pixel errors by more than 10 % (it was 1.01-1.23x
+lower over the same runs; training order on the device varies, so that
+measure moves most). The full
 mean is set by single caustic outliers at these counts, for both methods
 alike.
 
@@ -54,4 +55,4 @@ def test_guide_lowers_error_at_equal_samples():
           f"{tp:.5f} guided {tg:.5f} ({tp / tg:.2f}x)", flush=True)
     assert np.isfinite(guided).all()
     assert mg < 0.95 * mp
-    assert tg < 1.02 * tp
+    assert tg < 1.10 * tp
