@@ -115,6 +115,18 @@ struct xnh_ctx {
 
 namespace {
 
+// Default samples per pixel per batch: about 2^28 paths in flight (about
+// 75 GB of queues at 292 B per path, a quarter of the device), at most 256
+// samples. Each launch pays a fixed tail (its last, longest walks) and every
+// batch the kernel boundaries of all its bounces, so throughput keeps rising
+// with the batch: 1080p benchmark, 4 samples per batch (8.3 M paths) 559
+// Mpaths/s, 16 samples 834, 64 samples 960 (977 with later changes), 128
+// samples 1001.
+int default_nsamples(long long npix) {
+  const long long target = 1LL << 28;
+  return static_cast<int>(std::max(1LL, std::min(256LL, target / std::max(npix, 1LL))));
+}
+
 int fail(xnh_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
@@ -813,14 +825,7 @@ int xnh_create(const xnh_config* cfg, xnh_ctx** out) {
   if (cfg->spp_per_launch > 0) {
     c->nsamples = cfg->spp_per_launch;
   } else {
-    // paths in flight per batch: 2^28 (about 75 GB of queues at 292 B per
-    // path, a quarter of the device). Each launch pays a fixed tail (its
-    // last, longest walks) and every batch the kernel boundaries of all its
-    // bounces, so throughput keeps rising with the batch: 1080p benchmark,
-    // 4 samples per batch (8.3 M paths) 559 Mpaths/s, 16 samples 834,
-    // 64 samples 960 (977 with later changes), 128 samples 1001.
-    const long long target = 1LL << 28;
-    c->nsamples = static_cast<int>(std::max(1LL, std::min(256LL, target / npix)));
+    c->nsamples = default_nsamples(npix);
   }
   if (npix * c->nsamples > (1LL << 30)) {
     delete c;
@@ -1240,7 +1245,7 @@ int xnh_set_tuning(xnh_ctx* c, int32_t block_size, int32_t lds_stack, int32_t sp
     return fail(c, XNH_E_INVALID, "tuning out of range (block 64..256 in steps of 64, lds_stack 1..128)");
   const long long npix = static_cast<long long>(c->cfg.width) * c->cfg.height;
   int ns = spp_per_launch;
-  if (ns == 0) ns = static_cast<int>(std::max(1LL, std::min(64LL, (8LL << 20) / npix)));
+  if (ns == 0) ns = default_nsamples(npix);
   if (npix * ns > (1LL << 30)) return fail(c, XNH_E_INVALID, "queue capacity above 2^30 paths");
   if (c->guide.on && ns != c->nsamples) return fail(c, XNH_E_INVALID, "queue capacity is fixed while guiding is on");
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
