@@ -9,7 +9,16 @@ Each check is independent of the restated formula it covers:
     E_q[1/q] = 4 pi for the mixture sampler/pdf pair;
   * loss and back-propagation: float64 torch autograd of the same model;
   * Adam: the closed form of constant-gradient steps (each moves lr * sign(g)
-    once bias correction is applied).
+    once bias correction is applied);
+  * normal encoding (real spherical harmonics): scipy's complex Y_l^m
+    converted to the real basis, orthonormality by exact quadrature, and
+    the closed-form values at the poles;
+  * MLP forward: positive homogeneity of a bias-free ReLU network (bit for
+    bit, since doubling is exact in FP16 and FP32), identity and
+    permutation weights whose output is known without a matrix product, and
+    a one-hot input through non-negative weights, which reads back a row of
+    the product W1 W2 W3 W4;
+  * hash slots: the published primes evaluated with Python integers.
 """
 import math
 
@@ -262,3 +271,139 @@ def test_adam_constant_gradient_closed_form():
         # bias-corrected moments of a constant gradient are g and g^2
         assert np.allclose(p, p0 - t * 5e-3 * np.sign(g), atol=1e-5)
     # (without bias correction step 2 would already move 0.19 / sqrt(0.0199) = 1.35 lr)
+
+
+# ---- normal encoding: real spherical harmonics -----------------------------------------
+
+def _real_sh_scipy(nrm):
+    """Real SH of degree <= 3 from scipy's complex Y_l^m (Condon-Shortley
+    phase included): index l*l + l + m; m > 0: sqrt(2) Re Y_l^m, m < 0:
+    sqrt(2) Im Y_l^|m|, m = 0: Y_l^0."""
+    from scipy.special import sph_harm_y
+
+    theta = np.arccos(np.clip(nrm[:, 2].astype(np.float64), -1.0, 1.0))
+    phi = np.arctan2(nrm[:, 1].astype(np.float64), nrm[:, 0].astype(np.float64))
+    out = np.zeros((len(nrm), 16))
+    for l in range(4):
+        for m in range(-l, l + 1):
+            y = sph_harm_y(l, abs(m), theta, phi)
+            out[:, l * l + l + m] = y.real if m == 0 else math.sqrt(2.0) * (y.real if m > 0 else y.imag)
+    return out
+
+
+def test_sh4_matches_scipy_real_basis():
+    rng = np.random.default_rng(21)
+    nrm = _unit(rng, 5000).astype(np.float64)
+    got = G.sh4(nrm.astype(np.float32)).astype(np.float64)
+    want = _real_sh_scipy(nrm)
+    assert np.abs(got - want).max() < 2e-6
+
+
+def test_sh4_orthonormal_by_quadrature():
+    # Gauss-Legendre in z (16 nodes) x 32 equispaced azimuths integrates
+    # every polynomial of degree <= 6 in (x, y, z) on the sphere exactly
+    z, wz = np.polynomial.legendre.leggauss(16)
+    phi = (np.arange(32) + 0.5) * (2 * math.pi / 32)
+    Z, P = np.meshgrid(z, phi, indexing="ij")
+    r = np.sqrt(1 - Z * Z)
+    nrm = np.stack([r * np.cos(P), r * np.sin(P), Z], -1).reshape(-1, 3)
+    w = (wz[:, None] * np.full(32, 2 * math.pi / 32)[None, :]).reshape(-1)
+    Y = G.sh4(nrm.astype(np.float32)).astype(np.float64)
+    gram = (Y * w[:, None]).T @ Y
+    assert np.abs(gram - np.eye(16)).max() < 1e-5
+
+
+def test_sh4_pole_values():
+    for s in (1.0, -1.0):
+        y = G.sh4(np.array([[0.0, 0.0, s]], np.float32))[0].astype(np.float64)
+        for l in range(4):
+            for m in range(-l, l + 1):
+                want = s ** l * math.sqrt((2 * l + 1) / (4 * math.pi)) if m == 0 else 0.0
+                assert abs(y[l * l + l + m] - want) < 1e-6, (s, l, m)
+
+
+# ---- MLP forward --------------------------------------------------------------------------
+
+def test_mlp_forward_positive_homogeneity_bitwise():
+    rng = np.random.default_rng(31)
+    P = _params(3, 12)
+    enc = G.f16(rng.normal(0, 0.5, (300, G.N_IN)))
+    out1, acts1 = G.mlp_forward(P, enc)
+    out2, acts2 = G.mlp_forward(P, (enc * np.float32(2)).astype(np.float32))
+    # rounding to FP16 commutes with doubling except for FP16 subnormals
+    # (below 2^-14, whose spacing is fixed): rows holding one are compared
+    # by tolerance
+    sub = np.zeros(len(enc), bool)
+    for a in acts1[1:4]:
+        sub |= ((a > 0) & (a < 2.0 ** -14)).any(1)
+    assert (~sub).sum() >= 250
+    assert np.array_equal(out2[~sub], out1[~sub] * np.float32(2))
+    for a1, a2 in zip(acts1[1:4], acts2[1:4]):
+        assert np.array_equal(a2[~sub], a1[~sub] * np.float32(2))
+    assert np.allclose(out2[sub], out1[sub] * 2, rtol=1e-3, atol=1e-5)
+    # a negative scale is not a symmetry of ReLU: the map is not linear
+    out3, _ = G.mlp_forward(P, (enc * np.float32(-1)).astype(np.float32))
+    assert not np.allclose(out3, -out1)
+
+
+def test_mlp_forward_identity_and_permutation_weights():
+    rng = np.random.default_rng(32)
+    P = _params(4, 12)
+    sigma = rng.permutation(G.N_HID)
+    W1 = np.zeros((G.N_IN, G.N_HID), np.float32)
+    W1[np.arange(G.N_IN), np.arange(G.N_IN)] = 1.0
+    W2 = np.zeros((G.N_HID, G.N_HID), np.float32)
+    W2[np.arange(G.N_HID), sigma] = 1.0            # hidden unit i moves to sigma(i)
+    W3 = np.eye(G.N_HID, dtype=np.float32)
+    W4 = np.zeros((G.N_HID, G.N_OUT), np.float32)
+    W4[np.arange(G.N_OUT), np.arange(G.N_OUT)] = 1.0
+    P.W = [W1, W2, W3, W4]
+    enc = G.f16(rng.normal(0, 1.0, (50, G.N_IN)))
+    out, _ = G.mlp_forward(P, enc)
+    want = np.zeros((50, G.N_OUT), np.float32)
+    for i in range(G.N_IN):                      # input i -> hidden i -> hidden sigma(i) -> output sigma(i)
+        if sigma[i] < G.N_OUT:
+            want[:, sigma[i]] = np.maximum(enc[:, i], 0.0)
+    assert np.array_equal(out, want)
+
+
+def test_mlp_forward_one_hot_reads_row_of_weight_product():
+    rng = np.random.default_rng(33)
+    P = _params(5, 12)
+    P.W = [np.abs(W).astype(np.float32) for W in P.W]   # ReLUs stay inactive
+    prod = G.f16(P.W[0]).astype(np.float64)
+    for W in P.W[1:]:
+        prod = prod @ G.f16(W).astype(np.float64)
+    for i in (0, 7, 31, 47):
+        e = np.zeros((1, G.N_IN), np.float32)
+        e[0, i] = 1.0
+        out, _ = G.mlp_forward(P, e)
+        # three FP16 roundings of the hidden activations: relative 3 * 2^-11
+        assert np.allclose(out[0], prod[i], rtol=2e-3, atol=1e-6), i
+
+
+# ---- hash slots and the unit-cube map ------------------------------------------------------
+
+def test_hashed_corner_slots_python_integers():
+    rng = np.random.default_rng(41)
+    t = 14
+    res = 2048
+    c = rng.integers(0, res + 1, size=(500, 3))
+    got = G.corner_index(c[:, 0], c[:, 1], c[:, 2], res, t)
+    for k in range(len(c)):
+        x, y, z = (int(v) for v in c[k])
+        h = ((x * 1) ^ (y * 2654435761) ^ (z * 805459861)) % (1 << 32)
+        assert got[k] == h % (1 << t)
+    # a dense level is row-major x + n (y + n z)
+    d = G.corner_index(np.array([3]), np.array([5]), np.array([7]), 16, t)
+    assert d[0] == 3 + 17 * (5 + 17 * 7)
+
+
+def test_unit_pos_maps_box_to_unit_cube():
+    lo, hi = (-1.0, 2.0, 0.0), (3.0, 4.0, 0.0)   # a flat axis maps through a unit extent
+    p = np.array([[-1.0, 2.0, 0.0], [3.0, 4.0, 0.5], [1.0, 3.0, -1.0], [5.0, 1.0, 2.0]], np.float32)
+    q = G.unit_pos(p, lo, hi)
+    assert np.array_equal(q[0], [0.0, 0.0, 0.0])
+    assert np.array_equal(q[1], [G.POS_MAX, G.POS_MAX, 0.5])
+    assert np.array_equal(q[2], [0.5, 0.5, 0.0])
+    assert np.array_equal(q[3], [G.POS_MAX, 0.0, G.POS_MAX])
