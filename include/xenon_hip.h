@@ -301,7 +301,9 @@ int xnh_guide_train(xnh_ctx* ctx, int32_t n, const float* pos, const float* nrm,
  * traversal-stack entries per lane (1..128; deeper entries spill to HBM) and
  * samples per pixel per batch (sets the queue capacity; 0 = auto, about
  * 2^28 paths, at most 256 samples: 292 B per path, about 75 GB). Changing them reallocates the queues and clears the running
- * accumulation. */
+ * accumulation. If the new queues do not fit in device memory, the previous
+ * tuning is restored and the call returns XNH_E_ALLOC; if that too fails,
+ * renders return XNH_E_ALLOC until a later xnh_set_tuning succeeds. */
 typedef struct {
   int32_t block_size, lds_stack, spp_per_launch;
   float ms_per_spp;       /* device time of the winner per sample per pixel */
@@ -310,9 +312,14 @@ typedef struct {
 
 int xnh_set_tuning(xnh_ctx* ctx, int32_t block_size, int32_t lds_stack, int32_t spp_per_launch);
 int xnh_get_tuning(const xnh_ctx* ctx, int32_t* block_size, int32_t* lds_stack, int32_t* spp_per_launch);
-/* Times every candidate configuration on `probe_spp` samples of the
- * uploaded scene through `camera` and keeps the fastest. Clears the
- * accumulation. */
+/* Times candidate configurations on the uploaded scene through `camera`
+ * and keeps the fastest, in two stages: samples per batch (half, the
+ * configured value and double) at the configured workgroup size and LDS
+ * depth, then workgroup size (64, 128, 256) x LDS depth (8, 12, 16, 24, 32)
+ * at the chosen batch. Each candidate renders one untimed batch, then
+ * max(probe_spp, 2 x the stage's largest batch) timed samples. With guiding
+ * on, the batch size stays fixed and the network is not trained during the
+ * probes. Clears the accumulation. */
 int xnh_autotune(xnh_ctx* ctx, const xnh_camera* camera, int32_t probe_spp, xnh_tune_result* out);
 
 /* ---- output stage (main.cpp:25-52, math/image.h:11-182) ----------------

@@ -82,3 +82,78 @@ def test_tuning_does_not_change_results(tmp_path):
         assert again["cached"] and again["lds_stack"] == res["lds_stack"]
         r.render(hs.camera(), 1, 12, want_image=False)
         assert np.array_equal(r.read_accum()[0].view(np.uint32), outs[0].view(np.uint32))
+
+
+def test_autotune_centres_on_default_batch_and_is_not_slower():
+    """The batch-size candidates are half, the default (about 2^28 paths)
+    and double; the winner renders the probe no slower than the default
+    tuning does, within timing noise."""
+    from xenon_amd import HipRenderer, HipScene, scenes
+    desc = scenes.big_mesh(width=640, height=360, spp=8, level=5)
+    hs = HipScene(desc)
+    with HipRenderer(desc.width, desc.height, desc.max_bounces, desc.min_bounces) as r:
+        r.upload_scene(hs)
+        default = r.tuning()
+        print(f"[autotune] default {default}", flush=True)
+        res = r.autotune(hs.camera(), probe_spp=8, cache=None)
+        print(f"[autotune] chose {res}", flush=True)
+        assert res["spp_per_launch"] >= default["spp_per_launch"] // 2
+        assert res["candidates"] >= 2 + 15
+        probe = 4 * default["spp_per_launch"]
+
+        def timed():
+            r.render(hs.camera(), 1, default["spp_per_launch"], want_image=False)   # warm
+            best = 1e30
+            for _ in range(3):
+                r.render(hs.camera(), 1, probe, want_image=False)
+                best = min(best, r.stats()["render_ms"])
+            return best
+
+        t_win = timed()
+        r.set_tuning(0, 0, 0)
+        assert r.tuning() == default
+        t_def = timed()
+        print(f"[autotune] probe {probe} spp: winner {t_win:.1f} ms, default {t_def:.1f} ms", flush=True)
+        assert t_win <= 1.05 * t_def
+
+
+def test_autotune_with_guiding_keeps_capacity_and_network():
+    """With guiding on, autotune searches workgroup size and LDS depth at the
+    fixed queue capacity and does not train the network while probing."""
+    from xenon_amd import HipRenderer, HipScene, scenes
+    desc = scenes.cornell(64, 48, spp=8)
+    hs = HipScene(desc)
+    with HipRenderer(64, 48, 8, 3, spp_per_launch=4) as r:
+        r.upload_scene(hs)
+        r.guide_enable(seed=3)
+        r.render(hs.camera(), 1, 8, want_image=False)   # trains a few steps
+        g0, w0 = r.guide_params()
+        res = r.autotune(hs.camera(), probe_spp=4, cache=None)
+        print(f"[autotune guided] {res}", flush=True)
+        assert res["spp_per_launch"] == 4 and res["candidates"] == 1 + 15
+        g1, w1 = r.guide_params()
+        assert np.array_equal(g0, g1) and np.array_equal(w0, w1)
+        r.render(hs.camera(), 1, 8, want_image=False)   # guided rendering and training still work
+        assert r.stats()["paths"] == 64 * 48 * 8
+
+
+def test_set_tuning_that_does_not_fit_keeps_previous_tuning():
+    """A queue capacity the device cannot hold (2^30 paths at 292 B each is
+    more than an MI355X's 288 GB) fails with XNH_E_ALLOC; the previous
+    tuning is restored and renders as before."""
+    from xenon_amd import HipRenderer, HipScene, native, scenes
+    desc = scenes.cornell(1920, 1080, spp=2)
+    hs = HipScene(desc)
+    with HipRenderer(1920, 1080, 8, 3, spp_per_launch=1) as r:
+        r.upload_scene(hs)
+        r.render(hs.camera(), 1, 2, want_image=False)
+        ref = r.read_accum()[0]
+        before = r.tuning()
+        ns = (1 << 30) // (1920 * 1080)
+        with pytest.raises(native.XnhError) as e:
+            r.set_tuning(128, 8, ns)
+        print(f"[set_tuning] {e.value}", flush=True)
+        assert e.value.code == -6 and "previous tuning is kept" in str(e.value)
+        assert r.tuning() == before
+        r.render(hs.camera(), 1, 2, want_image=False)
+        assert np.array_equal(r.read_accum()[0].view(np.uint32), ref.view(np.uint32))

@@ -139,7 +139,9 @@ struct WaveBuffers {
   uint32_t* spill;  // traversal stack entries beyond the LDS part
   int32_t capacity;
   // path guiding (null / 0 when off)
-  float4* mix;      // [kMixVecs][capacity] mixture per queue entry (k_guide_infer)
+  float4* mix;      // [kMixVecs][mix_stride] mixture per queue entry (k_guide_infer)
+  uint32_t mix_stride;  // queue entries a guided batch uses at most (<= capacity): the stride of mix and of
+                        // the hash-grid encodings
   float4* rec;      // [4][rec_cap] training records, see guide.h
   uint32_t rec_cap;
   uint32_t rec_every;   // a path records its vertices when hash(slot, batch) % rec_every == 0

@@ -323,7 +323,7 @@ __device__ __forceinline__ f3 mixture_sample(const Mixture& m, float u0, float u
 
 // ---- stored mixtures (one per path-queue entry) --------------------------------
 //
-// mix[k * C + i], k = 0..11, for queue entry i of capacity C:
+// mix[k * C + i], k = 0..11, for queue entry i, C = WaveBuffers::mix_stride:
 //   k = 0..7   {mu_j.xyz, kappa_j}
 //   k = 8, 9   {pi_0..3}, {pi_4..7}
 //   k = 10, 11 pi_j * C(kappa_j), the pdf weight of lobe j

@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_encode(WaveBuffers B
   load_batch(v, B, counts, false);
   const uint32_t total = v.prefix[v.nch];
   const PathQueue q = B.q[qi];
-  const uint32_t C = static_cast<uint32_t>(B.capacity);
+  const uint32_t C = B.mix_stride;
   const uint32_t wid = threadIdx.x >> 6;
   for (uint32_t unit = blockIdx.x * kInferWaves + wid; unit < total; unit += gridDim.x * kInferWaves) {
     InferUnit u;
@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_mlp(WaveBuffers B, i
   load_batch(v, B, counts, false);
   stage_weights(net.wt, wl);
   const uint32_t total = v.prefix[v.nch];
-  const uint32_t C = static_cast<uint32_t>(B.capacity);
+  const uint32_t C = B.mix_stride;
   const uint32_t wid = threadIdx.x >> 6;
   _Float16* a = act[wid];
   uint4* row = reinterpret_cast<uint4*>(a + glane() * kGIn);

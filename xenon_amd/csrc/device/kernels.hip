@@ -30,7 +30,8 @@ namespace xnh {
 // ---------------------------------------------------------------------------
 // raygen: per-(pixel, sample) seed and jittered pinhole ray
 // (cpu_tiled.cpp:20-29, camera/camera.h:35-37); resets the batch's queue
-// counters and work cursors.
+// counters and work cursors. Grid-stride over the batch's paths, so the
+// grid is bounded whatever the queue capacity.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
   __shared__ BatchView v;
@@ -41,50 +42,52 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
   __syncthreads();
   if (static_cast<int>(threadIdx.x) < v.nch) v.ch[threadIdx.x] = B.chunks[v.batch * kMaxChunks + threadIdx.x];
   __syncthreads();
-  const int gid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (gid < v.nch) {
-    const DevChunk& ck = v.ch[gid];
-    B.counters[kCnt + gid] = B.jobs[ck.job].maxb > 0 ? static_cast<uint32_t>(ck.cap) : 0u;
-    B.counters[kCnt + kMaxChunks + gid] = 0u;
-    B.counters[kShCnt + gid] = 0u;
+  const int tid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (tid < v.nch) {
+    const DevChunk& ck = v.ch[tid];
+    B.counters[kCnt + tid] = B.jobs[ck.job].maxb > 0 ? static_cast<uint32_t>(ck.cap) : 0u;
+    B.counters[kCnt + kMaxChunks + tid] = 0u;
+    B.counters[kShCnt + tid] = 0u;
   }
-  if (gid < kNumCursors) B.counters[kCursor + gid] = 0u;
-  if (gid == 0) B.counters[kRecCnt] = 0u;
+  if (tid < kNumCursors) B.counters[kCursor + tid] = 0u;
+  if (tid == 0) B.counters[kRecCnt] = 0u;
   if (v.nch == 0) return;
-  const DevChunk& last = v.ch[v.nch - 1];
-  if (gid >= last.base + last.cap) return;
-  int lo = 0, hi = v.nch;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (v.ch[mid].base <= gid) lo = mid;
-    else hi = mid;
-  }
-  const DevChunk& ck = v.ch[lo];
-  const DevJob& J = B.jobs[ck.job];
-  const int local = gid - ck.base;
-  const int s = local / ck.npix;
-  const int pixel = local - s * ck.npix;
-  const int spp = ck.spp_first + s;
-  const uint64_t seed = static_cast<uint64_t>(pixel) + static_cast<uint64_t>(spp) * 0xdeadbeefULL;
-  Rng rng = rng_for(seed);
-  const int x = pixel % J.W, y = pixel / J.W;
-  const float fx = static_cast<float>(x) + rng.f();
-  const float fy = static_cast<float>(y) + rng.f();
-  const float u = fx / static_cast<float>(J.W), w = fy / static_cast<float>(J.H);
-  const f3 eye{J.cam.eye[0], J.cam.eye[1], J.cam.eye[2]};
-  const f3 llc{J.cam.llc[0], J.cam.llc[1], J.cam.llc[2]};
-  const f3 hz{J.cam.horiz[0], J.cam.horiz[1], J.cam.horiz[2]};
-  const f3 vt{J.cam.vert[0], J.cam.vert[1], J.cam.vert[2]};
-  const f3 d = normalize(llc + u * hz + w * vt - eye);
+  const int end = v.ch[v.nch - 1].base + v.ch[v.nch - 1].cap;
+  const int stride = static_cast<int>(gridDim.x * blockDim.x);
   const PathQueue& q = B.q[0];
-  q.ray0[gid] = make_float4(eye.x, eye.y, eye.z, d.x);
-  q.ray1[gid] = make_float4(d.y, d.z, 0.f, __uint_as_float(1u));
-  q.thr[gid] = make_float4(1.f, 1.f, 1.f, 0.f);
-  q.st[gid] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), static_cast<uint32_t>(gid),
-                         kNoRecord);
-  B.l_term[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
-  B.l_nee[2 * gid] = make_float4(0.f, 0.f, 0.f, 0.f);
-  B.l_nee[2 * gid + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int gid = tid; gid < end; gid += stride) {
+    int lo = 0, hi = v.nch;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (v.ch[mid].base <= gid) lo = mid;
+      else hi = mid;
+    }
+    const DevChunk& ck = v.ch[lo];
+    const DevJob& J = B.jobs[ck.job];
+    const int local = gid - ck.base;
+    const int s = local / ck.npix;
+    const int pixel = local - s * ck.npix;
+    const int spp = ck.spp_first + s;
+    const uint64_t seed = static_cast<uint64_t>(pixel) + static_cast<uint64_t>(spp) * 0xdeadbeefULL;
+    Rng rng = rng_for(seed);
+    const int x = pixel % J.W, y = pixel / J.W;
+    const float fx = static_cast<float>(x) + rng.f();
+    const float fy = static_cast<float>(y) + rng.f();
+    const float u = fx / static_cast<float>(J.W), w = fy / static_cast<float>(J.H);
+    const f3 eye{J.cam.eye[0], J.cam.eye[1], J.cam.eye[2]};
+    const f3 llc{J.cam.llc[0], J.cam.llc[1], J.cam.llc[2]};
+    const f3 hz{J.cam.horiz[0], J.cam.horiz[1], J.cam.horiz[2]};
+    const f3 vt{J.cam.vert[0], J.cam.vert[1], J.cam.vert[2]};
+    const f3 d = normalize(llc + u * hz + w * vt - eye);
+    q.ray0[gid] = make_float4(eye.x, eye.y, eye.z, d.x);
+    q.ray1[gid] = make_float4(d.y, d.z, 0.f, __uint_as_float(1u));
+    q.thr[gid] = make_float4(1.f, 1.f, 1.f, 0.f);
+    q.st[gid] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), static_cast<uint32_t>(gid),
+                           kNoRecord);
+    B.l_term[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    B.l_nee[2 * gid] = make_float4(0.f, 0.f, 0.f, 0.f);
+    B.l_nee[2 * gid + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -289,7 +292,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
           const float alpha = kGuided ? *B.alpha_now : 0.f;
           const uint32_t mi = (kGuided && depth == 0) ? static_cast<uint32_t>(ck.base) + off % static_cast<uint32_t>(ck.npix) : i;
           const bool guided = kGuided && depth < B.guide_depth && lambert_exact(m) &&
-                              B.mix[8 * B.capacity + mi].x >= 0.f;
+                              B.mix[8 * B.mix_stride + mi].x >= 0.f;
           // area-light NEE (cpu_tiled.cpp:88-139)
           if (S.num_lights > 0 && !delta) {
             const int nl = S.num_lights;
@@ -318,7 +321,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               if (cl > 1e-6f && wi_l.z > 1e-6f) {
                 const f3 f = nee_f(wo_l, wi_l, m);
                 float bp = nee_pdf(wo_l, wi_l, m);
-                if (guided) bp = alpha * stored_pdf(B.mix, B.capacity, mi, wi) + (1.f - alpha) * bp;
+                if (guided) bp = alpha * stored_pdf(B.mix, B.mix_stride, mi, wi) + (1.f - alpha) * bp;
                 const float lpdf = ((1.0f / l0.w) * pick) * (d2 / cl);
                 const float w = mis_power2(lpdf, bp);
                 const float cs = wi_l.z;
@@ -348,7 +351,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               if (wi_l.z > 1e-6f) {
                 const f3 f = nee_f(wo_l, wi_l, m);
                 float bp = nee_pdf(wo_l, wi_l, m);
-                if (guided) bp = alpha * stored_pdf(B.mix, B.capacity, mi, sdir) + (1.f - alpha) * bp;
+                if (guided) bp = alpha * stored_pdf(B.mix, B.mix_stride, mi, sdir) + (1.f - alpha) * bp;
                 const float cs = wi_l.z;
                 const float lpdf = 1e8f * spick;
                 const float w = mis_power2(lpdf, bp);
@@ -376,7 +379,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
                 const float u0 = rng.f();
                 const float u1 = rng.f();
                 const float u2 = rng.f();
-                wiw = stored_sample(B.mix, B.capacity, mi, u0, u1, u2);
+                wiw = stored_sample(B.mix, B.mix_stride, mi, u0, u1, u2);
                 wi_l = fs.local(wiw);
               } else {
                 BSample bs;
@@ -389,7 +392,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
               // of the shading normal (the side the BSDF sampler draws on)
               const bool same = (wo_l.z < 0.f) ? (wi_l.z < 0.f) : (wi_l.z > 0.f);
               const float ci2 = fabsf(wi_l.z);
-              const float pc = alpha * stored_pdf(B.mix, B.capacity, mi, wiw) + (1.f - alpha) * (same ? ci2 * kInvPiD : 0.f);
+              const float pc = alpha * stored_pdf(B.mix, B.mix_stride, mi, wiw) + (1.f - alpha) * (same ? ci2 * kInvPiD : 0.f);
               if (!same || ci2 < 1e-8f || !(pc >= 1e-8f)) {
                 alive = false;
               } else {
@@ -570,7 +573,8 @@ __global__ void __launch_bounds__(256) k_average(const float4* accum, float* out
 
 // ---------------------------------------------------------------------------
 // Ray-batch parity kernels (Scene::intersect / Scene::intersects).
-// rays: 2 float4 per ray {o, tmin} {d, tmax}; hits: 16 words (xnh_hit).
+// rays: 2 float4 per ray in the xnh_ray layout {o.xyz, d.x} {d.yz, tmin,
+// tmax}; hits: 16 words (xnh_hit).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_api_closest(DevScene S, const float4* rays, float* hits, int n,
                                                      uint32_t* spill, int lds_depth, unsigned long long* stats,
@@ -579,14 +583,15 @@ __global__ void __launch_bounds__(256) k_api_closest(DevScene S, const float4* r
   TravCount c{0, 0, 0, 0};
   const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   const bool valid = i < n;
-  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 1.f, 0.f);
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 1.f, 0.f, 0.f);
   if (valid) {
     a = rays[2 * i];
     b = rays[2 * i + 1];
   }
-  const f3 o{a.x, a.y, a.z}, d{b.x, b.y, b.z};
+  const f3 o{a.x, a.y, a.z}, d{a.w, b.x, b.y};
+  const float tmin = b.z, tmax = b.w;
   HitRef h{0.f, 0.f, 0.f, 0u, -1};
-  const bool ok = walk_scene<false>(S, o, d, a.w, b.w, valid, h, st, c);
+  const bool ok = walk_scene<false>(S, o, d, tmin, tmax, valid, h, st, c);
   if (valid) {
     float* out = hits + 16 * static_cast<size_t>(i);
     if (!ok) {
@@ -601,8 +606,8 @@ __global__ void __launch_bounds__(256) k_api_closest(DevScene S, const float4* r
     } else {
       float u = h.u, v = h.v;
       Surface sf;
-      if (h.slot & kPrimHit) sf = resolve_prim(S, h, load_inst(S.inst, h.inst), o, d, a.w, u, v);
-      else sf = resolve(S, h, o, d, a.w);
+      if (h.slot & kPrimHit) sf = resolve_prim(S, h, load_inst(S.inst, h.inst), o, d, tmin, u, v);
+      else sf = resolve(S, h, o, d, tmin);
       out[0] = h.t;
       out[1] = u;
       out[2] = v;
@@ -625,13 +630,13 @@ __global__ void __launch_bounds__(256) k_api_any(DevScene S, const float4* rays,
   TravCount c{0, 0, 0, 0};
   const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   const bool valid = i < n;
-  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 0.f, 1.f, 0.f);
+  float4 a = make_float4(0.f, 0.f, 0.f, 0.f), b = make_float4(0.f, 1.f, 0.f, 0.f);
   if (valid) {
     a = rays[2 * i];
     b = rays[2 * i + 1];
   }
   HitRef h{0.f, 0.f, 0.f, 0u, -1};
-  const bool hit = walk_scene<true>(S, f3{a.x, a.y, a.z}, f3{b.x, b.y, b.z}, a.w, b.w, valid, h, st, c);
+  const bool hit = walk_scene<true>(S, f3{a.x, a.y, a.z}, f3{a.w, b.x, b.y}, b.z, b.w, valid, h, st, c);
   if (valid) {
     occ[i] = hit ? 1 : 0;
     if (st.overflow) atomicAdd(overflow_ctr, 1u);
@@ -665,8 +670,11 @@ int num_cus() {
 }
 
 hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths) {
+  // at least kNumCursors threads (the counter resets), at most 4096
+  // work-groups (grid-stride beyond)
   const int n = max_paths > kNumCursors ? max_paths : kNumCursors;
-  hipLaunchKernelGGL(k_raygen, dim3((n + 255) / 256), dim3(256), 0, s, B);
+  const int blocks = (n + 255) / 256 < 4096 ? (n + 255) / 256 : 4096;
+  hipLaunchKernelGGL(k_raygen, dim3(blocks), dim3(256), 0, s, B);
   return hipGetLastError();
 }
 

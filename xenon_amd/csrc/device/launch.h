@@ -21,6 +21,8 @@ hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int gr
 hipError_t launch_accumulate(hipStream_t s, const WaveBuffers& B, int max_pixels);
 hipError_t launch_advance(hipStream_t s, uint32_t* counters);
 hipError_t launch_average(hipStream_t s, const float4* accum, float* out, int npix, float inv_spp);
+// Ray-batch kernels: rays in the xnh_ray layout (2 float4 per ray:
+// {org.xyz, dir.x} {dir.yz, tmin, tmax}), hits in the xnh_hit layout.
 hipError_t launch_api_closest(hipStream_t s, const DevScene& S, const float4* rays, float* hits, int n,
                               uint32_t* spill, int lds_depth, unsigned long long* stats, uint32_t* overflow_ctr);
 hipError_t launch_api_any(hipStream_t s, const DevScene& S, const float4* rays, uint8_t* occ, int n, uint32_t* spill,

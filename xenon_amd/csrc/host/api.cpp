@@ -70,6 +70,8 @@ struct GuideState {
   float* alpha_dev = nullptr;     // MIS probability of the guide for the next batch
   float alpha_host = 0.f;
   bool train = true;
+  uint32_t mix_stride = 0;        // queue entries of a guided batch (stride of mix and feat)
+  uint32_t rec_cap = 0, rec_every = 0;
 };
 
 struct xnh_ctx {
@@ -85,8 +87,9 @@ struct xnh_ctx {
   bool use_graph = true;
   xnh_gpu_scene own;              // the scene of xnh_upload_scene
   bool has_scene = false;
-  WaveBuffers B{};
+  WaveBuffers B{};                // B.capacity 0: no queues (a re-tuning failed twice)
   std::vector<void*> wave_allocs;
+  std::vector<void*> pix_allocs;  // accum and d_out: sized by the resolution, kept across re-tunings
   float4* accum = nullptr;        // the contract's running per-pixel sums
   DevJob* d_jobs = nullptr;
   DevChunk* d_chunks = nullptr;
@@ -458,6 +461,10 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   return XNH_OK;
 }
 
+// Queues for the current tuning (block, lds_depth, nsamples, grid_trace).
+// The previous queues are freed first: at the default batch one set is a
+// quarter of the device, so two sets need not fit together. On failure
+// nothing is left allocated and B.capacity is 0.
 int alloc_waves(xnh_ctx* c) {
   free_all(c->wave_allocs);
   const int npix = c->cfg.width * c->cfg.height;
@@ -485,12 +492,23 @@ int alloc_waves(xnh_ctx* c) {
     const size_t spill = static_cast<size_t>(kMaxStack - c->lds_depth) * c->grid_trace * c->block;
     rc = dev_alloc(c, c->wave_allocs, &B.spill, spill);
   }
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &c->accum, static_cast<size_t>(npix));
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &c->d_out, static_cast<size_t>(npix) * 3);
-  if (rc != XNH_OK) return rc;
-  HIP_OR_FAIL(c, hipMemset(c->accum, 0, sizeof(float4) * npix));
+  if (rc != XNH_OK) {
+    free_all(c->wave_allocs);
+    B = WaveBuffers{};
+    return rc;
+  }
   HIP_OR_FAIL(c, hipMemset(B.counters, 0, sizeof(uint32_t) * kNumCtr));
   HIP_OR_FAIL(c, hipMemset(B.stats, 0, sizeof(unsigned long long) * kNumStat));
+  return XNH_OK;
+}
+
+// The running per-pixel sums and the averaging buffer, allocated once.
+int alloc_pixels(xnh_ctx* c) {
+  const size_t npix = static_cast<size_t>(c->cfg.width) * c->cfg.height;
+  int rc = dev_alloc(c, c->pix_allocs, &c->accum, npix);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->pix_allocs, &c->d_out, npix * 3);
+  if (rc != XNH_OK) return rc;
+  HIP_OR_FAIL(c, hipMemset(c->accum, 0, sizeof(float4) * npix));
   return XNH_OK;
 }
 
@@ -655,13 +673,35 @@ struct JobSpec {
 // 2^22 of their paths; the full capacity trained too seldom.
 constexpr size_t kGuidedBatchPaths = size_t(1) << 26;
 
+// Queue entries a guided batch uses at most: kGuidedBatchPaths, or one
+// frame at the context's resolution if that is larger, within the queue
+// capacity. The guide's per-entry buffers are sized by it.
+size_t guided_cap(const xnh_ctx* c) {
+  const size_t npix = static_cast<size_t>(c->cfg.width) * c->cfg.height;
+  return std::min(static_cast<size_t>(c->B.capacity), std::max(kGuidedBatchPaths, npix));
+}
+
+// The guide's fields of the wavefront buffers (after the queues are
+// (re)allocated or guiding is switched on or off).
+void attach_guide(xnh_ctx* c) {
+  const GuideState& g = c->guide;
+  WaveBuffers& B = c->B;
+  B.mix = g.on ? g.mix : nullptr;
+  B.mix_stride = g.on ? g.mix_stride : 0u;
+  B.rec = g.on ? g.rec : nullptr;
+  B.rec_cap = g.on ? g.rec_cap : 0u;
+  B.rec_every = g.on ? g.rec_every : 0u;
+  B.guide_alpha = g.on ? g.cfg.guide_fraction : 0.f;
+  B.alpha_now = g.on ? g.alpha_dev : nullptr;
+  B.guide_depth = g.on ? g.cfg.guide_depth : 0;
+}
+
 int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
   size_t cap = static_cast<size_t>(c->B.capacity);
-  if (c->guide.on) {
-    size_t largest = 0;
-    for (const JobSpec& js : jobs) largest = std::max(largest, static_cast<size_t>(js.W) * js.H);
-    cap = std::min(cap, std::max(kGuidedBatchPaths, largest));
-  }
+  if (cap == 0)
+    return fail(c, XNH_E_ALLOC, "no wavefront queues: a re-tuning failed and the previous tuning could not be "
+                                "restored (a later xnh_set_tuning that succeeds makes the context usable again)");
+  if (c->guide.on) cap = c->guide.mix_stride;
   // batch schedule: greedy, one chunk per job per batch
   std::vector<DevChunk> chunks;
   std::vector<int32_t> counts;
@@ -677,7 +717,10 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
   for (size_t j = 0; j < jobs.size(); ++j) {
     const JobSpec& js = jobs[j];
     const size_t npix = static_cast<size_t>(js.W) * js.H;
-    if (npix > cap) return fail(c, XNH_E_INVALID, "job larger than the path capacity");
+    if (npix > cap)
+      return fail(c, XNH_E_INVALID, c->guide.on ? "job larger than a guided batch (the larger of 2^26 paths and one "
+                                                  "frame at the context's resolution)"
+                                                : "job larger than the path capacity");
     depth = std::max(depth, js.maxb);
     int remaining = js.spp_count, next = js.spp_first;
     while (remaining > 0) {
@@ -843,7 +886,8 @@ int xnh_create(const xnh_config* cfg, xnh_ctx** out) {
     return XNH_E_HIP;
   }
   c->grid_shade = cus * 8;
-  const int rc = alloc_waves(c);
+  int rc = alloc_pixels(c);
+  if (rc == XNH_OK) rc = alloc_waves(c);
   if (rc != XNH_OK) {
     xnh_destroy(c);
     return rc;
@@ -858,6 +902,7 @@ void xnh_destroy(xnh_ctx* c) {
   if (c->graph) hipGraphExecDestroy(c->graph);
   free_all(c->own.allocs);
   free_all(c->wave_allocs);
+  free_all(c->pix_allocs);
   free_all(c->guide.allocs);
   free_all(c->out.allocs);
   if (c->d_jobs) hipFree(c->d_jobs);
@@ -1251,23 +1296,28 @@ int xnh_set_tuning(xnh_ctx* c, int32_t block_size, int32_t lds_stack, int32_t sp
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   const int grid = trace_grid(block, lds);
   if (grid <= 0) return fail(c, XNH_E_HIP, "occupancy query failed");
-  WaveBuffers keep = c->B;
-  c->block = block;
-  c->lds_depth = lds;
-  c->nsamples = ns;
-  c->grid_trace = grid;
-  const int rc = alloc_waves(c);
-  if (rc != XNH_OK) return rc;
-  c->B.mix = keep.mix;
-  c->B.rec = keep.rec;
-  c->B.rec_cap = keep.rec_cap;
-  c->B.rec_every = keep.rec_every;
-  c->B.guide_alpha = keep.guide_alpha;
-  c->B.alpha_now = keep.alpha_now;
-  c->B.guide_depth = keep.guide_depth;
-  c->spp_total = 0;
+  // from here on the queues are reallocated and the accumulation cleared,
+  // whether or not the new tuning fits
+  const int old_block = c->block, old_lds = c->lds_depth, old_ns = c->nsamples, old_grid = c->grid_trace;
   c->table_gen++;   // the captured graph holds the old buffers
-  return XNH_OK;
+  c->spp_total = 0;
+  HIP_OR_FAIL(c, hipMemset(c->accum, 0, sizeof(float4) * static_cast<size_t>(npix)));
+  auto apply = [&](int b, int l, int n, int g) {
+    c->block = b;
+    c->lds_depth = l;
+    c->nsamples = n;
+    c->grid_trace = g;
+    const int r = alloc_waves(c);
+    attach_guide(c);
+    return r;
+  };
+  const int rc = apply(block, lds, ns, grid);
+  if (rc == XNH_OK) return XNH_OK;
+  const std::string why = c->err;
+  if (apply(old_block, old_lds, old_ns, old_grid) != XNH_OK)
+    return fail(c, rc, why + "; restoring the previous tuning failed too (" + c->err +
+                           "): the context renders nothing until a later xnh_set_tuning succeeds");
+  return fail(c, rc, why + " (the previous tuning is kept)");
 }
 
 int xnh_get_tuning(const xnh_ctx* c, int32_t* block_size, int32_t* lds_stack, int32_t* spp_per_launch) {
@@ -1281,8 +1331,12 @@ int xnh_get_tuning(const xnh_ctx* c, int32_t* block_size, int32_t* lds_stack, in
 int xnh_autotune(xnh_ctx* c, const xnh_camera* cam, int32_t probe_spp, xnh_tune_result* out) {
   if (!c || !cam || probe_spp <= 0) return XNH_E_INVALID;
   if (!c->has_scene) return fail(c, XNH_E_NO_SCENE, "xnh_autotune before xnh_upload_scene");
+  if (c->B.capacity == 0) return fail(c, XNH_E_ALLOC, "no wavefront queues (a re-tuning failed)");
   const long long npix = static_cast<long long>(c->cfg.width) * c->cfg.height;
-  const int ns0 = static_cast<int>(std::max(1LL, std::min(64LL, (8LL << 20) / npix)));
+  const int block0 = c->block, lds0 = c->lds_depth, ns0 = c->nsamples;
+  // samples per batch: half, the configured value (the default, about 2^28
+  // paths, unless set) and double; fixed while guiding (the guide's
+  // per-entry buffers are sized by the queues)
   std::vector<int> caps = {ns0};
   if (!c->guide.on) {
     if (ns0 / 2 >= 1) caps.push_back(ns0 / 2);
@@ -1291,28 +1345,40 @@ int xnh_autotune(xnh_ctx* c, const xnh_camera* cam, int32_t probe_spp, xnh_tune_
   const int blocks[] = {64, 128, 256};
   const int ldss[] = {8, 12, 16, 24, 32};
   float best = 3.0e38f;
-  int bb = c->block, bl = c->lds_depth, bn = c->nsamples, tried = 0;
+  int bb = block0, bl = lds0, bn = ns0, tried = 0;
   const bool prof = c->profiling;
+  const bool train = c->guide.train;
   c->profiling = false;
-  for (int ns : caps)
-    for (int b : blocks)
-      for (int l : ldss) {
-        int rc = xnh_set_tuning(c, b, l, ns);
-        if (rc != XNH_OK) continue;
-        // one untimed batch, then the probe
-        rc = xnh_render(c, cam, 1, std::min(ns, probe_spp), nullptr);
-        if (rc == XNH_OK) rc = xnh_render(c, cam, 1, probe_spp, nullptr);
-        if (rc != XNH_OK) continue;
-        ++tried;
-        const float ms = static_cast<float>(c->stats.render_ms) / static_cast<float>(probe_spp);
-        if (ms < best) {
-          best = ms;
-          bb = b;
-          bl = l;
-          bn = ns;
-        }
-      }
+  c->guide.train = false;   // probes leave the guide's parameters as they are
+  // Each candidate renders one untimed batch, then `probe` samples, at least
+  // two full batches of the stage's largest candidate, so that a larger
+  // batch is not timed on a partial one.
+  auto time_one = [&](int b, int l, int ns, int probe) {
+    if (xnh_set_tuning(c, b, l, ns) != XNH_OK) return;
+    int rc = xnh_render(c, cam, 1, ns, nullptr);
+    if (rc == XNH_OK) rc = xnh_render(c, cam, 1, probe, nullptr);
+    if (rc != XNH_OK) return;
+    ++tried;
+    const float ms = static_cast<float>(c->stats.render_ms) / static_cast<float>(probe);
+    if (ms < best) {
+      best = ms;
+      bb = b;
+      bl = l;
+      bn = ns;
+    }
+  };
+  // Stage 1: samples per batch at the configured workgroup size and LDS
+  // depth. Stage 2: workgroup size and LDS depth at the chosen batch. (The
+  // batch size moves throughput by up to 2x, the other two by about 1 %:
+  // searching them one after the other costs 3 + 15 probes instead of 45.)
+  const int ns_max = *std::max_element(caps.begin(), caps.end());
+  for (int ns : caps) time_one(block0, lds0, ns, std::max(probe_spp, 2 * ns_max));
+  const int ns_best = bn;
+  best = 3.0e38f;   // stage 2 times another probe length: compare within it
+  for (int b : blocks)
+    for (int l : ldss) time_one(b, l, ns_best, std::max(probe_spp, 2 * ns_best));
   c->profiling = prof;
+  c->guide.train = train;
   const int rc = xnh_set_tuning(c, bb, bl, bn);
   if (rc != XNH_OK) return rc;
   if (tried == 0) return fail(c, XNH_E_HIP, "no tuning candidate rendered");
@@ -1348,13 +1414,7 @@ int xnh_guide_disable(xnh_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   free_all(c->guide.allocs);
   c->guide = GuideState{};
-  c->B.mix = nullptr;
-  c->B.rec = nullptr;
-  c->B.rec_cap = 0;
-  c->B.rec_every = 0;
-  c->B.guide_alpha = 0.f;
-  c->B.alpha_now = nullptr;
-  c->B.guide_depth = 0;
+  attach_guide(c);
   c->table_gen++;
   return XNH_OK;
 }
@@ -1372,7 +1432,9 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (cfg.learning_rate <= 0.f) cfg.learning_rate = 5e-3f;
   if (cfg.guide_fraction <= 0.f) cfg.guide_fraction = 0.5f;
   if (cfg.train_batch <= 0) cfg.train_batch = 1 << 18;
-  if (cfg.record_paths <= 0 || cfg.record_paths > c->B.capacity) cfg.record_paths = c->B.capacity;
+  if (c->B.capacity == 0) return fail(c, XNH_E_ALLOC, "no wavefront queues (a re-tuning failed)");
+  const size_t gcap = guided_cap(c);
+  if (cfg.record_paths <= 0 || static_cast<size_t>(cfg.record_paths) > gcap) cfg.record_paths = static_cast<int32_t>(gcap);
   // one optimisation step per 2^22 paths a guided batch holds: the guide
   // then learns at the same rate per rendered sample whatever the batch
   // size. Caustic room, 2^26-path batches, equal time against plain: 1
@@ -1401,13 +1463,12 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_h, size_t(kWTotal));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_t, size_t(kWTotal));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.loss, size_t(1));
-  const size_t cap = static_cast<size_t>(c->B.capacity);
   const size_t R = static_cast<size_t>(cfg.train_batch);
   // records: one per guided vertex of a recording path
   const size_t RC = std::min(static_cast<size_t>(cfg.record_paths) * static_cast<size_t>(cfg.guide_depth),
                              static_cast<size_t>(1) << 25);
-  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.mix, size_t(kMixVecs) * cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.feat, size_t(6) * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.mix, size_t(kMixVecs) * gcap);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.feat, size_t(6) * gcap);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.rec, 4 * RC);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.lam_all, RC);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.lam_sum_all, size_t(1));
@@ -1425,13 +1486,9 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   g.net.grid = g.grid_h;
   g.net.w = g.w_h;
   g.net.wt = g.w_t;
-  c->B.mix = g.mix;
-  c->B.rec = g.rec;
-  c->B.rec_cap = static_cast<uint32_t>(RC);
-  c->B.rec_every = static_cast<uint32_t>(std::max<size_t>(1, cap / static_cast<size_t>(cfg.record_paths)));
-  c->B.guide_alpha = cfg.guide_fraction;
-  c->B.alpha_now = g.alpha_dev;
-  c->B.guide_depth = cfg.guide_depth;
+  g.mix_stride = static_cast<uint32_t>(gcap);
+  g.rec_cap = static_cast<uint32_t>(RC);
+  g.rec_every = static_cast<uint32_t>(std::max<size_t>(1, gcap / static_cast<size_t>(cfg.record_paths)));
   g.alpha_host = cfg.warmup_steps > 0 ? 0.f : cfg.guide_fraction;
   if (hipMemcpy(g.alpha_dev, &g.alpha_host, sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
     xnh_guide_disable(c);
@@ -1458,6 +1515,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
     for (int i = 0; i < dims[L][0] * dims[L][1]; ++i) w[offs[L] + i] = a * uni();
   }
   g.on = true;
+  attach_guide(c);
   return xnh_guide_set_params(c, grid.data(), w.data());
 }
 
@@ -1612,11 +1670,9 @@ static int trace_common(xnh_ctx* c, const xnh_ray* rays, int32_t n, bool any, vo
   if (!c->has_scene) return fail(c, XNH_E_NO_SCENE, "trace before xnh_upload_scene");
   c->stats = xnh_stats{};
   if (n == 0) return XNH_OK;
+  static_assert(sizeof(xnh_ray) == 2 * sizeof(float4), "the kernels read xnh_ray as two float4");
   std::vector<float4> hr(2 * static_cast<size_t>(n));
-  for (int i = 0; i < n; ++i) {
-    hr[2 * i] = f4(rays[i].org[0], rays[i].org[1], rays[i].org[2], rays[i].tmin);
-    hr[2 * i + 1] = f4(rays[i].dir[0], rays[i].dir[1], rays[i].dir[2], rays[i].tmax);
-  }
+  std::memcpy(hr.data(), rays, sizeof(xnh_ray) * static_cast<size_t>(n));
   std::vector<void*> tmp;
   float4* d_rays = nullptr;
   void* d_out = nullptr;
