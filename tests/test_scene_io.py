@@ -342,3 +342,39 @@ def test_written_instancing_scene_round_trip(tmp_path):
     for m in range(len(desc.meshes)):
         for a, b in zip(built.blas(m), hs.blas(m)):
             assert a.tobytes() == b.tobytes()
+
+
+# ---------------------------------------------------------------------------
+# the reference-side binding (xenon_amd/binding/hip_renderer.cpp)
+# ---------------------------------------------------------------------------
+
+def test_reference_binding_flattens_like_the_library(reference, tmp_path):
+    """The binding a maintainer adds to the reference (src/hip/hip_renderer.cpp,
+    kept as xenon_amd/binding/hip_renderer.cpp) compiled against the
+    reference's own headers (oracle/Makefile target `binding`): for every
+    sample scene file, the edge-case scene and a written description, the
+    xnh_flat_scene it makes from the reference's Scene (load_scene +
+    build_acceleration) equals, array by array and byte for byte, the one
+    the library makes from the same file (xnh_scene_load_file +
+    xnh_scene_flat). Host code only; no GPU call."""
+    import subprocess
+    oracle_dir = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    subprocess.run(["make", "-C", oracle_dir, "-s", "binding"], check=True, capture_output=True)
+    exe = os.path.join(oracle_dir, "_ref", "check_binding")
+    paths = []
+    for rel in _ref_scene_files():
+        d, f = rel.split("/")
+        shutil.copytree(os.path.join(REF_SCENES, d), tmp_path / "scenes" / d,
+                        ignore=shutil.ignore_patterns("*.py"), dirs_exist_ok=True)
+        paths.append(str(tmp_path / "scenes" / d / f))
+    SF.write_reference_meshes(str(tmp_path))
+    paths.append(_edge_scene(tmp_path))
+    desc = scenes.cornell(64, 48)
+    desc.suns.append(scenes.Sun((0.2, 1.0, 0.1), (1.0, 0.9, 0.8), 2.0))
+    (tmp_path / "written").mkdir()
+    paths.append(SF.write_scene(desc, str(tmp_path / "written"), with_normals=True))
+    r = subprocess.run([exe, *paths], cwd=str(tmp_path), capture_output=True, text=True, timeout=600)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert sum(line.startswith("OK ") for line in r.stdout.splitlines()) == len(paths)
+    assert len(paths) >= 10
