@@ -227,27 +227,157 @@ def _cpu_baseline(desc, budget_s: float):
         osc.render_sum(1, n, threads)
     dt = time.perf_counter() - t0
     return {"value": round(W * H * n / dt / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": kind,
+            "cpu_model": cpu_model(),
             "sample": f"{n} frame(s) of the same scene at {W}x{H}, 1 sample per pixel each ({W * H * n} paths, "
                       f"{dt:.1f} s), max depth {desc.max_bounces}; paths are independent so Mpaths/s is "
                       f"per-path throughput"}
 
 
+def simd_use(tot: dict) -> dict:
+    """Active-lane fractions of the traversal phases, from the counting
+    re-run: lane steps over 64 x wave iterations of the interior phase (node
+    pairs), the leaf phase (held leaves) and the round loop (lanes holding a
+    ray), with the node pairs and leaf steps per ray."""
+    def frac(a, b):
+        return round(tot[a] / (64.0 * tot[b]), 4) if tot.get(b) else None
+
+    out = {}
+    for pre, rays in (("", "closest_rays"), ("shadow_", "shadow_rays")):
+        out[(pre or "closest_") + "interior"] = frac(pre + "interior", pre + "interior_iters")
+        out[(pre or "closest_") + "leaf"] = frac(pre + "leaf_steps", pre + "leaf_iters")
+        out[(pre or "closest_") + "round"] = frac(pre + "round_lanes", pre + "round_iters")
+        if tot.get(rays):
+            out[(pre or "closest_") + "pairs_per_ray"] = round(tot[pre + "interior"] / tot[rays], 3)
+            out[(pre or "closest_") + "leaf_steps_per_ray"] = round(tot[pre + "leaf_steps"] / tot[rays], 3)
+            out[(pre or "closest_") + "wave_iters_per_ray"] = round(
+                (tot[pre + "interior_iters"] + tot[pre + "leaf_iters"]) * 64.0 / tot[rays], 3)
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+CONFIG_NAMES = {0: "cornell", 1: "big_mesh", 2: "instancing", 3: "batched_jobs"}
+
+
+class Workload:
+    """One configuration set up for timing: `step(k)` renders step k's
+    samples (distinct sample indices per step), `paths` per step."""
+
+    def __init__(self, name, renderer, step, paths, config, data, cpu_descs, extra=None):
+        self.name, self.r, self.step, self.paths = name, renderer, step, paths
+        self.config, self.data, self.cpu_descs, self.extra = config, data, cpu_descs, extra or {}
+
+
+def setup_single(args, desc, label):
+    """Configurations 0-2: one scene, `sps` samples of every pixel per step."""
+    from xenon_amd import HipRenderer, HipScene
+
+    t0 = time.perf_counter()
+    hs = HipScene(desc)
+    log(f"scene: {desc.num_tris} triangles, built in {time.perf_counter() - t0:.1f} s")
+    r = HipRenderer(desc.width, desc.height, desc.max_bounces, desc.min_bounces,
+                    spp_per_launch=args.spp_per_launch, block_size=args.block, lds_stack=args.lds_stack)
+    t0 = time.perf_counter()
+    r.upload_scene(hs)
+    cam = hs.camera()
+    log(f"upload {time.perf_counter() - t0:.2f} s")
+    sps = args.spp_per_step or desc.spp
+
+    def step(k):
+        r.render(cam, 1 + k * sps, sps, want_image=False)
+
+    cfg = {"workload": f"{label}, {desc.width}x{desc.height}, {sps} spp per step, depth {desc.max_bounces}, "
+                       f"RR from {desc.min_bounces}",
+           "width": desc.width, "height": desc.height, "spp_per_step": sps, "triangles": desc.num_tris}
+    return Workload(label, r, step, desc.width * desc.height * sps, cfg, "synthetic (seeded scene generator)",
+                    [(desc, 1.0)])
+
+
+def setup_jobs(args):
+    """Configuration 3: the 32 jobs of the seeded job list (seed 6), each on
+    the full-size scene of its generator (Cornell, the L8 displaced
+    icosphere, the 4096-instance scene; one device-resident copy of each,
+    shared by the jobs of that generator, each job with its own camera
+    aspect), all rendered by one xnh_render_jobs call per step."""
+    from xenon_amd import HipRenderer, HipScene, scenes
+    from xenon_amd.renderer import GpuScene
+
+    jobs = scenes.job_list()
+    gens = {0: scenes.cornell(512, 512), 1: scenes.big_mesh(), 2: scenes.instancing()}
+    t0 = time.perf_counter()
+    hss = {k: HipScene(d) for k, d in gens.items()}
+    log(f"job scenes built in {time.perf_counter() - t0:.1f} s")
+    r = HipRenderer(1920, 1080, 12, 3, spp_per_launch=args.spp_per_launch, block_size=args.block,
+                    lds_stack=args.lds_stack)
+    r.upload_scene(hss[1])
+    tuned = None
+    if args.autotune:
+        t0 = time.perf_counter()
+        tuned = r.autotune(hss[1].camera(), cache=None)
+        log(f"autotune {time.perf_counter() - t0:.1f} s: {tuned}")
+    gs = {k: GpuScene(r, h) for k, h in hss.items()}
+    spec = []
+    for j in jobs:
+        d = gens[j.kind]
+        spec.append(dict(scene=gs[j.kind], camera=hss[j.kind].camera(j.width, j.height), width=j.width,
+                         height=j.height, spp_count=j.spp, max_bounces=j.max_bounces, min_bounces=d.min_bounces))
+    paths = sum(j.width * j.height * j.spp for j in jobs)
+
+    def step(k):
+        for sj, j in zip(spec, jobs):
+            sj["spp_first"] = 1 + k * j.spp
+        r.render_jobs(spec, want_images=False)
+
+    kinds = [sum(1 for j in jobs if j.kind == g) for g in range(3)]
+    cfg = {"workload": f"{len(jobs)} batched jobs (seeded list, seed 6): {kinds[0]} Cornell, {kinds[1]} displaced "
+                       f"icosphere L8 ({gens[1].num_tris} tris), {kinds[2]} instancing (4096 instances, "
+                       f"{gens[2].num_tris} unique tris); resolutions 256^2-2048^2, spp 16-1024, depth 4-12; one "
+                       f"render_jobs call per step",
+           "jobs": len(jobs), "paths_per_step": paths,
+           "tuning": tuned if tuned is not None else r.tuning()}
+    # CPU sample: each generator's scene at the resolution of its first job,
+    # weighted by that generator's share of the job list's paths
+    w = [sum(j.width * j.height * j.spp for j in jobs if j.kind == g) / paths for g in range(3)]
+    cpu = []
+    for g in range(3):
+        first = next((j for j in jobs if j.kind == g), None)
+        if first is None:
+            continue
+        d = {0: scenes.cornell, 1: scenes.big_mesh, 2: scenes.instancing}[g](width=first.width, height=first.height,
+                                                                           max_bounces=first.max_bounces)
+        cpu.append((d, w[g]))
+    return Workload("batched_jobs", r, step, paths, cfg, "synthetic (seeded scene generators and job list)", cpu,
+                    extra={"gpu_scenes": gs})
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--config", type=int, default=1, choices=[0, 1, 2, 3],
+                    help="0 Cornell 512^2 64 spp; 1 displaced icosphere 1080p 256 spp (the default line); "
+                         "2 instancing 1080p 64 spp; 3 the 32 batched jobs")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 3; 10 for config 0; 1 for 3)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spp-per-step", type=int, default=256)
+    ap.add_argument("--spp-per-step", type=int, default=0, help="0: the configuration's spp")
     ap.add_argument("--level", type=int, default=8, help="icosphere subdivision level (8 = the benchmark)")
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp-per-launch", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--lds-stack", type=int, default=0)
+    ap.add_argument("--autotune", type=int, default=1, help="config 3: run the autotuner first (1) or not (0)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--guided-seconds", type=float, default=8.0,
-                    help="wall-clock budget of each side of the guided-vs-plain comparison (0: skip)")
+                    help="config 1: wall-clock budget of each side of the guided-vs-plain comparison (0: skip)")
     ap.add_argument("--guided-width", type=int, default=1280)
     ap.add_argument("--guided-height", type=int, default=720)
     ap.add_argument("--guided-ref-spp", type=int, default=16384)
@@ -260,6 +390,8 @@ def main():
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the renderer shards by "
               f"samples across GPUs by design but this build runs on one device).", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = {0: 10, 1: 3, 2: 3, 3: 1}[args.config]
 
     import torch
 
@@ -274,36 +406,36 @@ def main():
                           "launches": 5}), flush=True)
         return
 
-    from xenon_amd import HipRenderer, HipScene, scenes
+    from xenon_amd import scenes
 
-    t0 = time.perf_counter()
-    desc = scenes.big_mesh(width=args.width, height=args.height, level=args.level)
-    hs = HipScene(desc)
-    log(f"scene: {desc.num_tris} triangles, built in {time.perf_counter() - t0:.1f} s")
-    r = HipRenderer(desc.width, desc.height, desc.max_bounces, desc.min_bounces,
-                    spp_per_launch=args.spp_per_launch, block_size=args.block, lds_stack=args.lds_stack)
-    t0 = time.perf_counter()
-    r.upload_scene(hs)
-    cam = hs.camera()
-    log(f"upload {time.perf_counter() - t0:.2f} s")
-    sps = args.spp_per_step
-    spp = 1
+    if args.config == 0:
+        wl = setup_single(args, scenes.cornell(512, 512, 64), "Cornell box (36 tris, diffuse, area light)")
+    elif args.config == 1:
+        wl = setup_single(args, scenes.big_mesh(width=args.width, height=args.height, level=args.level),
+                          f"displaced icosphere L{args.level} + ground, GGX conductor r=0.3, sun+sky")
+    elif args.config == 2:
+        desc = scenes.instancing()
+        wl = setup_single(args, desc, f"instancing: TLAS over 4096 instances of 10 displaced-icosphere BLAS "
+                                      f"({desc.num_tris} unique tris), 40/30/30 diffuse/GGX/dielectric, "
+                                      f"64 emissive tris")
+    else:
+        wl = setup_jobs(args)
+    r = wl.r
     for w in range(args.warmup):
-        r.render(cam, spp, sps, want_image=False)
-        spp += sps
+        wl.step(args.steps + w)   # sample indices after the timed steps'
         log(f"warmup step {w + 1}/{args.warmup}: {r.stats()['render_ms']:.1f} ms")
     r.set_profiling(timing=True)
     keys = ["closest_rays", "shadow_rays", "nonfinite", "closest_ms", "shadow_ms", "shade_ms", "closest_launches",
             "render_ms"]
     work_keys = ["node_tests", "interior", "tri_tests", "inst_visits", "shadow_node_tests", "shadow_interior",
-                 "shadow_tri_tests", "shadow_inst_visits"]
+                 "shadow_tri_tests", "shadow_inst_visits", "interior_iters", "leaf_iters", "leaf_steps",
+                 "round_iters", "round_lanes", "shadow_interior_iters", "shadow_leaf_iters", "shadow_leaf_steps",
+                 "shadow_round_iters", "shadow_round_lanes"]
     tot = {k: 0 for k in keys + work_keys}
-    first_timed = spp
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
-        r.render(cam, spp, sps, want_image=False)
-        spp += sps
+        wl.step(k)
         st = r.stats()
         for key in keys:
             tot[key] += st[key]
@@ -316,22 +448,23 @@ def main():
     # identical); untimed.
     r.set_profiling(work=True)
     for k in range(args.steps):
-        r.render(cam, first_timed + k * sps, sps, want_image=False)
+        wl.step(k)
         st = r.stats()
         for key in work_keys:
             tot[key] += st[key]
     r.set_profiling()
-    paths = desc.width * desc.height * sps * args.steps
+    paths = wl.paths * args.steps
     mpaths = paths / elapsed / 1e6
     mrays = (tot["closest_rays"] + tot["shadow_rays"]) / elapsed / 1e6
     cbytes = algorithmic_bytes(tot, "closest")
     launches = max(1, tot["closest_launches"])
     achieved = cbytes / (tot["closest_ms"] / 1e3) / 1e9
-    traffic = read_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic() if args.config == 1 else None,
                 "kernel": "k_closest (closest-hit traversal)",
                 "bytes_per_launch": round(cbytes / launches), "avg_launch_ms": round(tot["closest_ms"] / launches, 4)}
+    sbytes = algorithmic_bytes(tot, "shadow")
     out = {
         "metric": METRIC,
         "value": round(mpaths, 3),
@@ -344,30 +477,54 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded displaced icosphere scene)",
-        "config": {"workload": f"displaced icosphere L{args.level} ({desc.num_tris} tris) + ground, GGX conductor "
-                               f"r=0.3, sun+sky, {desc.width}x{desc.height}, {sps} spp per step, depth "
-                               f"{desc.max_bounces}, RR from {desc.min_bounces}",
-                   "width": desc.width, "height": desc.height, "spp_per_step": sps, "triangles": desc.num_tris},
+        "data": wl.data,
+        "config": {"config": args.config, "name": CONFIG_NAMES[args.config], **wl.config},
         "mrays_per_s": round(mrays, 3),
         "rays_per_path": round((tot["closest_rays"] + tot["shadow_rays"]) / paths, 3),
         "kernel_ms": {"closest": round(tot["closest_ms"], 2), "shade": round(tot["shade_ms"], 2),
                       "shadow": round(tot["shadow_ms"], 2), "device_total": round(tot["render_ms"], 2)},
+        "shadow_roofline": {"achieved": round(sbytes / (tot["shadow_ms"] / 1e3) / 1e9, 2) if tot["shadow_ms"] else None,
+                            "frac": round(sbytes / (tot["shadow_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                            if tot["shadow_ms"] else None, "unit": "GB/s", "kernel": "k_shadow (any-hit traversal)"},
         "nonfinite_paths": int(tot["nonfinite"]),
+        "simd": simd_use(tot),
+        "work_per_ray": {k: round(tot[k] / max(1, tot[r]), 3) for k, r in (
+            ("node_tests", "closest_rays"), ("tri_tests", "closest_rays"), ("inst_visits", "closest_rays"),
+            ("shadow_node_tests", "shadow_rays"), ("shadow_tri_tests", "shadow_rays"),
+            ("shadow_inst_visits", "shadow_rays"))},
         "roofline": roofline,
     }
+    for g in wl.extra.get("gpu_scenes", {}).values():
+        g.release()
     r.close()
-    if args.guided_seconds > 0:
+    if args.config == 1 and args.guided_seconds > 0:
         out["guided"] = guided_comparison(args.guided_seconds, args.guided_width, args.guided_height,
                                           args.guided_ref_spp, args.guided_freq)
     if not args.no_cpu:
         log("timing the CPU integrator ...")
-        cb = cpu_baseline(desc, args.cpu_seconds)
+        cb = cpu_baseline_weighted(wl.cpu_descs, args.cpu_seconds)
         cb["mpaths_ratio_gpu_over_cpu"] = round(mpaths / cb["value"], 1) if cb["value"] else None
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_weighted(descs, budget_s: float):
+    """The CPU baseline of one scene, or of several scenes weighted by their
+    share of the workload's paths (batched jobs): the weighted harmonic mean
+    of their rates, i.e. total paths / total CPU time."""
+    if len(descs) == 1:
+        return cpu_baseline(descs[0][0], budget_s)
+    parts = [(cpu_baseline(d, budget_s / len(descs)), w) for d, w in descs]
+    wsum = sum(w for _, w in parts)
+    rate = wsum / sum(w / p["value"] for p, w in parts)
+    first = parts[0][0]
+    return {"value": round(rate, 4), "unit": "Mpaths/s", "cores": first["cores"], "kind": first["kind"],
+            "cpu_model": first["cpu_model"],
+            "sample": "per generator, the scene of its first job at that job's resolution and depth, timed as: " +
+                      " | ".join(f"weight {w:.3f}: {p['value']} Mpaths/s over {p['sample']}" for p, w in parts) +
+                      "; combined as total paths over total CPU time"}
 
 
 if __name__ == "__main__":

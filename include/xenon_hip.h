@@ -206,6 +206,13 @@ typedef struct {
   double shade_ms;           /* device time inside the shading kernels */
   int32_t closest_launches;  /* closest-hit kernel launches in the call */
   int32_t stack_overflows;   /* lanes whose traversal stack exceeded its limit */
+  /* SIMD use of the traversal, with the work counters (XNH_PROFILE_WORK and
+   * the ray-batch calls; zero otherwise): wave iterations of the interior
+   * phase, of the leaf phase and of the round loop, lane-level leaf steps
+   * and rounds. A phase's active-lane fraction is its lane steps (interior,
+   * leaf_steps) over 64 x its wave iterations. Closest hit, then any hit. */
+  uint64_t interior_iters, leaf_iters, leaf_steps, round_iters, round_lanes;
+  uint64_t shadow_interior_iters, shadow_leaf_iters, shadow_leaf_steps, shadow_round_iters, shadow_round_lanes;
 } xnh_stats;
 
 /* ---- renderer (CudaRenderer contract) ---------------------------------- */

@@ -50,6 +50,8 @@ void orc_trace_any(void* scene, int n, const float* rays8, uint8_t* occluded, ui
 int orc_render(void* scene, int W, int H, int spp_first, int spp_count, int min_bounces, int max_bounces,
                int nthreads, float* accum, uint64_t* stats3);
 void orc_path(void* scene, int W, int H, int x, int y, int spp, int min_bounces, int max_bounces, float* out3);
+int orc_path_rays(void* scene, int W, int H, int x, int y, int spp, int min_bounces, int max_bounces, int max_rays,
+                  float* out11);
 void orc_pcg_stream(uint64_t seed, uint64_t seq, int n, uint32_t* out);
 void orc_bsdf_sample(const float* params16, const float* wo3, uint64_t seed, int n, float* out9);
 void orc_bsdf_eval(const float* params16, int n, const float* wo3, const float* wi3, float* out4);

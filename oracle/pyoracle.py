@@ -310,6 +310,18 @@ class OScene:
                               self.desc.min_bounces, self.desc.max_bounces, out.ctypes.data_as(_fp))
         return out
 
+    def path_rays(self, x, y, spp, width=None, height=None, max_rays=64):
+        """The closest-hit rays of one path (restatement only): (n, 8) rays
+        (o, d, tmin, tmax) and (n, 3) hits (t, instance, triangle)."""
+        fn = self.lib.dll.orc_path_rays
+        fn.restype = C.c_int
+        fn.argtypes = [_vp] + [C.c_int] * 8 + [_fp]
+        out = np.zeros((max_rays, 11), np.float32)
+        n = fn(self.h, width or self.desc.width, height or self.desc.height, x, y, spp, self.desc.min_bounces,
+               self.desc.max_bounces, max_rays, out.ctypes.data_as(_fp))
+        n = min(n, max_rays)
+        return out[:n, :8].copy(), out[:n, 8:].copy()
+
 
 # ---- scene files: the reference's own load_scene (oracle/_ref only) --------
 _FILE_SIGS = {

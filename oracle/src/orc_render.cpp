@@ -80,6 +80,9 @@ struct Scene {
 
 struct PathStats {
   uint64_t closest = 0, shadow = 0, nonfinite = 0;
+  // optional log of the path's closest-hit rays (o, d, tmin, tmax) and
+  // their hits (t, instance, triangle; -1 ids on a miss), for diagnosis
+  std::vector<float>* ray_log = nullptr;
 };
 
 // One path for pixel (x, y) and 1-based sample index `spp`
@@ -99,7 +102,14 @@ static F3 radiance(const Scene& sc, int x, int y, int W, int H, int spp, int min
   for (int depth = 0; depth < maxb; ++depth) {
     HitF rec;
     st.closest++;
-    if (!sc.tlas.closest(ray, rec, sc.blas, nullptr)) {
+    const bool hit = sc.tlas.closest(ray, rec, sc.blas, nullptr);
+    if (st.ray_log) {
+      const float e[11] = {ray.o.x, ray.o.y, ray.o.z, ray.d.x, ray.d.y, ray.d.z, ray.tmin, ray.tmax,
+                           hit ? rec.t : kInf, hit ? static_cast<float>(rec.inst) : -1.f,
+                           hit ? static_cast<float>(rec.prim) : -1.f};
+      st.ray_log->insert(st.ray_log->end(), e, e + 11);
+    }
+    if (!hit) {
       if (sc.sky.intensity > 0.f) L = L + T * sc.sky.eval(ray.d);
       break;
     }
@@ -472,6 +482,20 @@ void orc_path(void* h, int W, int H, int x, int y, int spp, int minb, int maxb, 
   PathStats st;
   const F3 L = radiance(*S(h), x, y, W, H, spp, minb, maxb, st);
   out3[0] = L.x; out3[1] = L.y; out3[2] = L.z;
+}
+
+// The closest-hit rays of one path and their hits, 11 floats each (o, d,
+// tmin, tmax, t, instance, triangle); returns the ray count (at most
+// max_rays are written).
+int orc_path_rays(void* h, int W, int H, int x, int y, int spp, int minb, int maxb, int max_rays, float* out) {
+  PathStats st;
+  std::vector<float> log;
+  st.ray_log = &log;
+  radiance(*S(h), x, y, W, H, spp, minb, maxb, st);
+  const int n = static_cast<int>(log.size() / 11);
+  for (int k = 0; k < n && k < max_rays; ++k)
+    for (int j = 0; j < 11; ++j) out[11 * k + j] = log[11 * static_cast<size_t>(k) + j];
+  return n;
 }
 
 // ---- isolated-function hooks for known-answer tests --------------------------

@@ -2,10 +2,13 @@
 identical rays. Integer results (instance, triangle, material ids, facing,
 occlusion) must be bit-identical; float results are compared as raw bits and
 must be identical too, apart from a grazing-edge fraction that the tolerance
-below bounds at 1e-5. The device walk visits nodes speculatively (a lane
-keeps walking interior nodes while a leaf waits for the wave's leaf phase),
-which can only add work: instance visits must equal the oracle's, node and
-triangle tests may exceed them by a bounded margin, never fall short."""
+below bounds at 1e-5. The device's closest-hit walk tests every box and
+triangle the reference's recursive walk tests, in its order and against the
+same t_best, so its work counters (nodes, interior expansions, triangles,
+instance visits) must equal the oracle's exactly. The scenes include the
+benchmark geometries at full size: the 1,310,722-triangle displaced
+icosphere (L8), the 4096-instance scene over ten 20k-200k-triangle BLAS and
+the caustic room's 499,280-triangle glass sphere."""
 import numpy as np
 import pytest
 
@@ -24,10 +27,14 @@ def _scenes():
         "big_mesh": lambda: scenes.big_mesh(width=64, height=36, level=6),
         "instancing": lambda: scenes.instancing(width=64, height=36, n_inst=512, tri_range=(2000, 20000), n_blas=10,
                                                 n_emit=64),
+        "big_mesh_L8": lambda: scenes.big_mesh(width=64, height=36),
+        "instancing_4096": lambda: scenes.instancing(width=64, height=36),
+        "caustic_room": lambda: scenes.caustic_room(width=64, height=36),
     }
 
 
-@pytest.fixture(scope="module", params=["cornell", "big_mesh", "instancing"])
+@pytest.fixture(scope="module", params=["cornell", "big_mesh", "instancing", "big_mesh_L8", "instancing_4096",
+                                        "caustic_room"])
 def pair(request, oracle):
     from xenon_amd import HipRenderer, HipScene
     print(f"[trace parity] scene {request.param}", flush=True)

@@ -175,6 +175,20 @@ enum Stat : int {
   kStShTriTests,
   kStShInstVisits,
   kStNonFinite,
+  // SIMD use of the traversal (counting variants only): wave iterations of
+  // the interior and leaf phases and of the persistent round loop (counted
+  // once per wave), and the lane-level leaf steps and rounds; closest hit,
+  // then any hit
+  kStIntIters,
+  kStLeafIters,
+  kStLeafSteps,
+  kStRoundIters,
+  kStRoundLanes,
+  kStShIntIters,
+  kStShLeafIters,
+  kStShLeafSteps,
+  kStShRoundIters,
+  kStShRoundLanes,
   kNumStat
 };
 

@@ -120,7 +120,7 @@ __device__ __forceinline__ f3 fresnel_s(float c, f3 f0) {
 __device__ __forceinline__ f3 cosine_dir(float u1, float u2) {
   const float r = sqrtf(u1);
   const float phi = 2.0f * kPiD * u2;
-  return f3{r * cosf(phi), r * sinf(phi), sqrtf(smax(0.0f, 1.0f - u1))};
+  return f3{r * cos_r(phi), r * sin_r(phi), sqrtf(smax(0.0f, 1.0f - u1))};
 }
 
 // Heitz 2018 visible-normal sampling (bsdf.h:164-190).
@@ -131,8 +131,8 @@ __device__ __forceinline__ f3 vndf(f3 wo, float ax, float ay, float u1, float u2
   const f3 t2 = cross(vh, t1);
   const float r = sqrtf(u1);
   const float phi = 2.0f * kPiD * u2;
-  const float p1 = r * cosf(phi);
-  float p2 = r * sinf(phi);
+  const float p1 = r * cos_r(phi);
+  float p2 = r * sin_r(phi);
   const float s = 0.5f * (1.0f + vh.z);
   p2 = (1.0f - s) * sqrtf(smax(0.f, 1.0f - p1 * p1)) + s * p2;
   const f3 nh = p1 * t1 + p2 * t2 + sqrtf(smax(0.f, 1.0f - p1 * p1 - p2 * p2)) * vh;
@@ -147,7 +147,7 @@ __device__ __forceinline__ float vndf_pdf(f3 wo, f3 h, float ax, float ay) {
 }
 
 __device__ __forceinline__ float ms_E(float c, float a) {
-  const float e = 1.0f - powf(1.0f - c, 5.0f * expf(-2.69f * a)) / (1.0f + 22.7f * powf(a, 1.5f));
+  const float e = 1.0f - pow_r(1.0f - c, 5.0f * exp_r(-2.69f * a)) / (1.0f + 22.7f * pow_r(a, 1.5f));
   return sclamp(e, 0.f, 1.f);
 }
 __device__ __forceinline__ f3 ms_comp(f3 f0, float co, float ci, float a) {

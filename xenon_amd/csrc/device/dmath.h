@@ -31,6 +31,21 @@ __device__ __forceinline__ void gst1(uint32_t* p, size_t i, uint32_t v) {
   ((__attribute__((address_space(1))) uint32_t*)p)[i] = v;
 }
 
+// Float cos, sin, pow and exp evaluated in double and rounded once to
+// float: the correctly rounded result except in rare double-rounding cases.
+// The reference calls glibc's float functions (std::cos(float) etc. in
+// material/bsdf.h and scene/environment.h), which compute in double
+// internally and return the correctly rounded value for nearly every input;
+// the device library's float versions differ from it in the last bit far
+// more often, and a last-bit difference in a sampled direction can send a
+// path elsewhere.
+__device__ __attribute__((noinline)) float cos_r(float x) { return static_cast<float>(cos(static_cast<double>(x))); }
+__device__ __attribute__((noinline)) float sin_r(float x) { return static_cast<float>(sin(static_cast<double>(x))); }
+__device__ __attribute__((noinline)) float pow_r(float x, float y) {
+  return static_cast<float>(pow(static_cast<double>(x), static_cast<double>(y)));
+}
+__device__ __attribute__((noinline)) float exp_r(float x) { return static_cast<float>(exp(static_cast<double>(x))); }
+
 __device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
 __device__ __forceinline__ f3 splat(float s) { return f3{s, s, s}; }
 __device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }

@@ -72,7 +72,15 @@ struct TravStack {
 // for; the timed variant carries none of them.
 struct TravCount {
   uint32_t node_tests, interior, tri_tests, inst_visits;
+  // SIMD use: int_iters, leaf_iters and round_iters count wave iterations
+  // (on one lane of the wave); leaf_steps and round_lanes count lanes
+  uint32_t int_iters, leaf_iters, leaf_steps, round_iters, round_lanes;
 };
+
+// True on the lowest active lane of the wave.
+__device__ __forceinline__ bool first_active_lane() {
+  return static_cast<int>(threadIdx.x & 63u) == __ffsll(static_cast<long long>(__ballot(1))) - 1;
+}
 
 struct HitRef {
   float t, u, v;
@@ -659,6 +667,7 @@ struct Walker {
     for (;;) {
       const bool walk = leaf == 0u && ent_interior(e);
       if (__ballot(walk) == 0ull) break;
+      if (kCount && first_active_lane()) ++c.int_iters;
       if (walk) interior_step<kFast>(S, st, c);
       if (st.overflow) break;
     }
@@ -677,6 +686,8 @@ struct Walker {
     for (;;) {
       const bool has = leaf != 0u;
       if (__ballot(has) == 0ull) break;
+      if (kCount && first_active_lane()) ++c.leaf_iters;
+      if (kCount && has) ++c.leaf_steps;
       if (has && leaf_step(S, st, c)) return true;
     }
     if (e == kEntEnd && leaf == 0u) end_instance();

@@ -44,19 +44,31 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 // serial tail. Every thread of the work-group must reach it (256 threads).
 __device__ __forceinline__ void flush_counts_block(unsigned long long* stats, const TravCount& c, uint32_t rays,
                                                    bool any, bool work) {
-  __shared__ unsigned long long red[5][256 / 64];
+  constexpr int kN = 10;
+  __shared__ unsigned long long red[kN][256 / 64];
   const int w = static_cast<int>(threadIdx.x >> 6), nw = static_cast<int>(blockDim.x >> 6);
-  const unsigned long long v[5] = {wave_sum(rays), work ? wave_sum(c.node_tests) : 0ull,
-                                   work ? wave_sum(c.interior) : 0ull, work ? wave_sum(c.tri_tests) : 0ull,
-                                   work ? wave_sum(c.inst_visits) : 0ull};
+  const unsigned long long v[kN] = {
+      wave_sum(rays),
+      work ? wave_sum(c.node_tests) : 0ull,
+      work ? wave_sum(c.interior) : 0ull,
+      work ? wave_sum(c.tri_tests) : 0ull,
+      work ? wave_sum(c.inst_visits) : 0ull,
+      work ? wave_sum(c.int_iters) : 0ull,
+      work ? wave_sum(c.leaf_iters) : 0ull,
+      work ? wave_sum(c.leaf_steps) : 0ull,
+      work ? wave_sum(c.round_iters) : 0ull,
+      work ? wave_sum(c.round_lanes) : 0ull};
   if (lane_id() == 0)
-    for (int k = 0; k < 5; ++k) red[k][w] = v[k];
+    for (int k = 0; k < kN; ++k) red[k][w] = v[k];
   __syncthreads();
-  if (threadIdx.x < 5u) {
+  if (threadIdx.x < static_cast<unsigned>(kN)) {
     unsigned long long t = 0;
     for (int k = 0; k < nw; ++k) t += red[threadIdx.x][k];
-    const int base = any ? kStShNodeTests : kStNodeTests;
-    const int slot = threadIdx.x == 0 ? (any ? kStShadow : kStClosest) : base + static_cast<int>(threadIdx.x) - 1;
+    const int k = static_cast<int>(threadIdx.x);
+    int slot;
+    if (k == 0) slot = any ? kStShadow : kStClosest;
+    else if (k < 5) slot = (any ? kStShNodeTests : kStNodeTests) + k - 1;
+    else slot = (any ? kStShIntIters : kStIntIters) + k - 5;
     if (t) atomicAdd(&stats[slot], t);
   }
 }
@@ -75,6 +87,12 @@ __device__ __forceinline__ void flush_counts(unsigned long long* stats, const Tr
     if (t) atomicAdd(&stats[base + 2], t);
     if (i) atomicAdd(&stats[base + 3], i);
   }
+  const unsigned long long p[5] = {wave_sum(c.int_iters), wave_sum(c.leaf_iters), wave_sum(c.leaf_steps),
+                                   wave_sum(c.round_iters), wave_sum(c.round_lanes)};
+  const int pbase = any ? kStShIntIters : kStIntIters;
+  if (lane_id() == 0)
+    for (int k = 0; k < 5; ++k)
+      if (p[k]) atomicAdd(&stats[pbase + k], p[k]);
 }
 
 __device__ __forceinline__ TravStack make_stack(uint32_t* spill, int lds_depth) {
@@ -188,7 +206,7 @@ __device__ __forceinline__ f3 sky_eval(const DevSky& k, f3 d) {
     const float t = smin(-d.y, 1.f);
     return k.intensity * lerp3(hz, f3{k.ground[0], k.ground[1], k.ground[2]}, t);
   }
-  const float t = powf(d.y, 1.0f / k.sharpness);
+  const float t = pow_r(d.y, 1.0f / k.sharpness);
   return k.intensity * lerp3(hz, f3{k.zenith[0], k.zenith[1], k.zenith[2]}, t);
 }
 

@@ -158,7 +158,9 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
       }
     }
     if (__ballot(my != kIdle) == 0ull) break;
+    if (kCount && lane == 0) ++c.round_iters;
     if (my != kIdle) {
+      if (kCount) ++c.round_lanes;
       const DevScene& S = B.jobs[uniform(v.ch[pci].job)].S;
       if (w.round(S, st, c)) {
         overflow |= st.overflow;

@@ -130,6 +130,19 @@ int default_nsamples(long long npix) {
   return static_cast<int>(std::max(1LL, std::min(256LL, target / std::max(npix, 1LL))));
 }
 
+void copy_simd_stats(xnh_stats& o, const unsigned long long* st) {
+  o.interior_iters = st[kStIntIters];
+  o.leaf_iters = st[kStLeafIters];
+  o.leaf_steps = st[kStLeafSteps];
+  o.round_iters = st[kStRoundIters];
+  o.round_lanes = st[kStRoundLanes];
+  o.shadow_interior_iters = st[kStShIntIters];
+  o.shadow_leaf_iters = st[kStShLeafIters];
+  o.shadow_leaf_steps = st[kStShLeafSteps];
+  o.shadow_round_iters = st[kStShRoundIters];
+  o.shadow_round_lanes = st[kStShRoundLanes];
+}
+
 int fail(xnh_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
   return code;
@@ -142,13 +155,25 @@ int fail(xnh_ctx* c, int code, const std::string& msg) {
       return fail((ctx), XNH_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
+// hipMalloc that reports a refusal as XNH_E_ALLOC and clears the runtime's
+// last-error state it leaves, which the next launch check (hipGetLastError)
+// would otherwise report as a failure of that launch.
+int raw_alloc(xnh_ctx* c, void** p, size_t bytes, const char* what) {
+  *p = nullptr;
+  const hipError_t e = hipMalloc(p, bytes);
+  if (e == hipSuccess) return XNH_OK;
+  (void)hipGetLastError();
+  *p = nullptr;
+  return fail(c, XNH_E_ALLOC, std::string(what) + ": hipMalloc: " + hipGetErrorString(e));
+}
+
 template <class T>
 int dev_alloc(xnh_ctx* c, std::vector<void*>& owner, T** out, size_t count) {
   *out = nullptr;
   if (count == 0) count = 1;
   void* p = nullptr;
-  const hipError_t e = hipMalloc(&p, count * sizeof(T));
-  if (e != hipSuccess) return fail(c, XNH_E_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+  const int rc = raw_alloc(c, &p, count * sizeof(T), "device buffer");
+  if (rc != XNH_OK) return rc;
   owner.push_back(p);
   *out = static_cast<T*>(p);
   return XNH_OK;
@@ -518,9 +543,12 @@ int ensure_tables(xnh_ctx* c, size_t njobs, size_t nbatches) {
   if (njobs > c->jobs_cap) {
     if (c->d_jobs) hipFree(c->d_jobs);
     c->d_jobs = nullptr;
+    c->jobs_cap = 0;
+    c->table_gen++;
     size_t cap = std::max<size_t>(njobs, 64);
     void* p = nullptr;
-    if (hipMalloc(&p, cap * sizeof(DevJob)) != hipSuccess) return fail(c, XNH_E_ALLOC, "job table");
+    const int rc = raw_alloc(c, &p, cap * sizeof(DevJob), "job table");
+    if (rc != XNH_OK) return rc;
     c->d_jobs = static_cast<DevJob*>(p);
     c->jobs_cap = cap;
     c->table_gen++;
@@ -530,11 +558,15 @@ int ensure_tables(xnh_ctx* c, size_t njobs, size_t nbatches) {
     if (c->d_nchunks) hipFree(c->d_nchunks);
     c->d_chunks = nullptr;
     c->d_nchunks = nullptr;
+    c->batches_cap = 0;
+    c->table_gen++;
     size_t cap = std::max<size_t>(nbatches, 256);
     void *p = nullptr, *q = nullptr;
-    if (hipMalloc(&p, cap * kMaxChunks * sizeof(DevChunk)) != hipSuccess) return fail(c, XNH_E_ALLOC, "chunk table");
+    int rc = raw_alloc(c, &p, cap * kMaxChunks * sizeof(DevChunk), "chunk table");
+    if (rc != XNH_OK) return rc;
     c->d_chunks = static_cast<DevChunk*>(p);
-    if (hipMalloc(&q, cap * sizeof(int32_t)) != hipSuccess) return fail(c, XNH_E_ALLOC, "chunk counts");
+    rc = raw_alloc(c, &q, cap * sizeof(int32_t), "chunk counts");
+    if (rc != XNH_OK) return rc;
     c->d_nchunks = static_cast<int32_t*>(q);
     c->batches_cap = cap;
     c->table_gen++;
@@ -822,6 +854,7 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
   o.shadow_tri_tests = st[kStShTriTests];
   o.shadow_inst_visits = st[kStShInstVisits];
   o.nonfinite = st[kStNonFinite];
+  copy_simd_stats(o, st);
   float ms = 0.f;
   HIP_OR_FAIL(c, hipEventElapsedTime(&ms, c->ev_begin, c->ev_end));
   o.render_ms = ms;
@@ -1079,15 +1112,13 @@ int output_source(xnh_ctx* c, const float* rgb, int w, int h) {
 
 }  // namespace
 
-int xnh_image_noise_stats(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, xnh_noise_stats* out) {
-  if (!c || !out || w < 0 || h < 0) return XNH_E_INVALID;
-  *out = xnh_noise_stats{};
-  if (w <= 2 || h <= 2) return XNH_OK;   // image.h:41-42
-  int rc = output_source(c, rgb, w, h);
-  if (rc != XNH_OK) return rc;
-  HIP_OR_FAIL(c, launch_noise_stats(c->stream, c->out.img, w, h, c->out.sc));
+namespace {
+
+// Reads the device state of a finished NoiseStats computation into `out`
+// (math/image.h:100-118).
+int read_noise_stats(xnh_ctx* c, const ImgState* dst, int32_t w, int32_t h, xnh_noise_stats* out) {
   ImgState st;
-  HIP_OR_FAIL(c, hipMemcpyAsync(&st, c->out.sc.st, sizeof st, hipMemcpyDeviceToHost, c->stream));
+  HIP_OR_FAIL(c, hipMemcpyAsync(&st, dst, sizeof st, hipMemcpyDeviceToHost, c->stream));
   HIP_OR_FAIL(c, hipStreamSynchronize(c->stream));
   const int interior = (w - 2) * (h - 2);
   out->total_interior_pixel_count = interior;
@@ -1106,6 +1137,19 @@ int xnh_image_noise_stats(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, xn
   }
   return XNH_OK;
 }
+
+}  // namespace
+
+int xnh_image_noise_stats(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, xnh_noise_stats* out) {
+  if (!c || !out || w < 0 || h < 0) return XNH_E_INVALID;
+  *out = xnh_noise_stats{};
+  if (w <= 2 || h <= 2) return XNH_OK;   // image.h:41-42
+  int rc = output_source(c, rgb, w, h);
+  if (rc != XNH_OK) return rc;
+  HIP_OR_FAIL(c, launch_noise_stats(c->stream, c->out.img, w, h, c->out.sc));
+  return read_noise_stats(c, c->out.sc.st, w, h, out);
+}
+
 
 int xnh_image_rgb8(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, uint8_t* out) {
   if (!c || !out || w < 0 || h < 0) return XNH_E_INVALID;
@@ -1725,6 +1769,7 @@ static int trace_common(xnh_ctx* c, const xnh_ray* rays, int32_t n, bool any, vo
   c->stats.shadow_interior = st[kStShInterior];
   c->stats.shadow_tri_tests = st[kStShTriTests];
   c->stats.shadow_inst_visits = st[kStShInstVisits];
+  copy_simd_stats(c->stats, st);
   c->stats.render_ms = ms;
   c->stats.stack_overflows = static_cast<int32_t>(ovf);
   if (ovf) return fail(c, XNH_E_OVERFLOW, "traversal stack overflow");

@@ -115,7 +115,12 @@ class Stats(C.Structure):
                 ("inst_visits", C.c_uint64), ("shadow_node_tests", C.c_uint64), ("shadow_interior", C.c_uint64),
                 ("shadow_tri_tests", C.c_uint64), ("shadow_inst_visits", C.c_uint64), ("nonfinite", C.c_uint64), ("render_ms", C.c_double),
                 ("closest_ms", C.c_double), ("shadow_ms", C.c_double), ("shade_ms", C.c_double),
-                ("closest_launches", C.c_int32), ("stack_overflows", C.c_int32)]
+                ("closest_launches", C.c_int32), ("stack_overflows", C.c_int32),
+                ("interior_iters", C.c_uint64), ("leaf_iters", C.c_uint64), ("leaf_steps", C.c_uint64),
+                ("round_iters", C.c_uint64), ("round_lanes", C.c_uint64),
+                ("shadow_interior_iters", C.c_uint64), ("shadow_leaf_iters", C.c_uint64),
+                ("shadow_leaf_steps", C.c_uint64), ("shadow_round_iters", C.c_uint64),
+                ("shadow_round_lanes", C.c_uint64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -388,22 +388,25 @@ class HipRenderer:
         self._chk(N.lib().xnh_get_stats(self.h, C.byref(s)))
         return s.as_dict()
 
-    def render_jobs(self, jobs, want_sums: bool = False):
+    def render_jobs(self, jobs, want_sums: bool = False, want_images: bool = True):
         """Renders a list of jobs together. Each job is a dict with keys
         scene (GpuScene or None for the uploaded scene), camera, width,
         height, spp_first, spp_count, max_bounces, min_bounces. Returns one
-        (H*W, 3) average per job (and the sums when want_sums)."""
+        (H*W, 3) average per job (and the sums when want_sums); with
+        want_images=False and no sums nothing is copied back and the
+        averages are None."""
         arr = (N.Job * len(jobs))()
         outs, sums = [], []
         for k, j in enumerate(jobs):
             W, H = int(j["width"]), int(j["height"])
-            o = np.empty((W * H, 3), np.float32)
+            o = np.empty((W * H, 3), np.float32) if want_images else None
             sm = np.empty((W * H, 3), np.float32) if want_sums else None
             outs.append(o)
             sums.append(sm)
             sc = j.get("scene")
             arr[k] = N.Job(sc.h if sc is not None else None, j["camera"], W, H, int(j.get("spp_first", 1)),
-                           int(j["spp_count"]), int(j["max_bounces"]), int(j["min_bounces"]), _fp(o),
+                           int(j["spp_count"]), int(j["max_bounces"]), int(j["min_bounces"]),
+                           _fp(o) if o is not None else None,
                            _fp(sm) if sm is not None else None)
         self._chk(N.lib().xnh_render_jobs(self.h, arr, len(jobs)))
         return (outs, sums) if want_sums else outs
