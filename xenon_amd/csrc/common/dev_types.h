@@ -50,12 +50,13 @@ constexpr uint32_t kNoRoot = 0xFFFFFFFFu;
 // flag that marks a primitive in a hit's slot (slot = primitive index).
 constexpr uint32_t kPrimRoot = 0xFFFFFFFEu;
 constexpr uint32_t kPrimHit = 0x80000000u;
-// Bound on the walk rounds of one wave in a launch: far above any
-// legitimate walk (a wave's share of a batch takes ~1e5 rounds), it makes
-// every wave exit even if a round stopped making progress; a wave that
-// reaches it abandons its work and is reported. (Within a round every loop
-// ends: the host rejects trees whose children do not follow their parent.)
-constexpr uint32_t kWaveRoundLimit = 1u << 26;
+// Bound on the traversal iterations of one wave in a launch: far above any
+// legitimate walk (a wave's share of a 2^28-path batch of the 4096-instance
+// scene takes some 3e5 iterations), it makes every wave exit even if its
+// walks stopped making progress; a wave that reaches it abandons its work
+// and is reported. (Every iteration advances at least one lane by one step,
+// and the host rejects trees whose children do not follow their parent.)
+constexpr uint32_t kWaveStepLimit = 1u << 26;
 
 struct DevSky {
   float zenith[3], horizon[3], ground[3];
@@ -79,12 +80,29 @@ struct DevScene {
   const float4* prims;      // 3 x float4 per primitive: {type, mat, p0, p1} {p2..p5} {p6..p9}
   const float4* pnodes;     // 2 x float4 per PrimBLAS node, as bnodes without the pair layout
   const uint32_t* porder;   // PrimBLAS leaf slot -> primitive index
+  const uint32_t* ikind;    // per instance: the shading class of all its surfaces (kShade*), for sorting
   int32_t num_pnodes;
   int32_t num_tnodes;
   int32_t num_lights;
   int32_t num_suns;
   int32_t pad;
   DevSky sky;
+};
+
+// Shading classes: which branch of the shading kernel a hit takes, so that
+// a wave can be given hits of one class (kernels.hip k_shade). An instance's
+// class is its materials' common class, or kShadeMixed.
+enum ShadeClass : uint32_t {
+  kShadeMiss = 0,
+  kShadeLambert,      // opaque, non-metal, F0 = 0: one Lambert lobe
+  kShadeConductor,
+  kShadeDielectric,   // opaque dielectric: GGX and Lambert lobes
+  kShadeRoughGlass,
+  kShadeDelta,
+  kShadeSubsurface,
+  kShadeMixed,        // mixed classes, or analytic primitives
+  kShadeInvalid,      // no entry (past the end of the queue)
+  kNumShadeClasses
 };
 
 struct DevCamera {

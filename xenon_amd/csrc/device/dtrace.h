@@ -413,7 +413,7 @@ struct Walker {
   uint32_t leaf;      // held leaf entry (tested before e), 0 = none
   bool inst_found, found;
   bool fast;          // lo, ld, linv finite: pair_test_fast is exact
-  bool stuck;         // the round bound was reached (the walk was abandoned)
+  bool stuck;         // the step bound was reached (the walk was abandoned)
   HitRef hit;         // u, v, slot, inst of the closest hit; its t is tbest
 
   __device__ __forceinline__ void begin(const DevScene& S, f3 o_, f3 d_, float tmin_, float tmax_, TravStack& st) {
@@ -512,84 +512,91 @@ struct Walker {
     return false;
   }
 
-  // TLAS steps until this lane is inside a BLAS walk; true = ray finished.
-  __device__ __forceinline__ bool tlas_steps(const DevScene& S, TravStack& st, TravCount& c) {
-    while (blas_base < 0) {
-      if (st.sp == 0) return true;
-      const uint32_t te = st.pop();
-      if (te & kInstTag) {
-        const int ii = static_cast<int>(te & ~kInstTag);
-        const InstRec in = load_inst(S.inst, ii);
-        if (kCount) ++c.inst_visits;
-        lo = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
-        ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
-        linv = f3{1.f / ld.x, 1.f / ld.y, 1.f / ld.z};
-        fast = all_finite(linv) && all_finite(ld) && all_finite(lo);
-        inst = ii;
-        inst_found = false;
-        if (in.root == kPrimRoot) {   // the primitives: walked here, to the end
-          if (kPrims) {
-            if (prim_walk(S, st, c)) {
-              found = true;
-              return true;
-            }
-            if (!kAny && inst_found) {
-              hit.inst = ii;
-              found = true;
-            }
-            if (st.overflow) return true;
+  // One TLAS step: pops one TLAS entry, a node (tested on pop, its children
+  // or instances pushed) or an instance to enter (its BLAS walk starts, or
+  // the primitives are walked to the end). True once the ray is finished.
+  __device__ __forceinline__ bool tlas_step(const DevScene& S, TravStack& st, TravCount& c) {
+    if (st.sp == 0) return true;
+    const uint32_t te = st.pop();
+    if (te & kInstTag) {
+      const int ii = static_cast<int>(te & ~kInstTag);
+      const InstRec in = load_inst(S.inst, ii);
+      if (kCount) ++c.inst_visits;
+      lo = in.c0 * o.x + in.c1 * o.y + in.c2 * o.z + in.c3;
+      ld = in.c0 * d.x + in.c1 * d.y + in.c2 * d.z;
+      linv = f3{1.f / ld.x, 1.f / ld.y, 1.f / ld.z};
+      fast = all_finite(linv) && all_finite(ld) && all_finite(lo);
+      inst = ii;
+      inst_found = false;
+      if (in.root == kPrimRoot) {   // the primitives: walked here, to the end
+        if (kPrims) {
+          if (prim_walk(S, st, c)) {
+            found = true;
+            return true;
           }
-          continue;
-        }
-        blas_base = st.sp;
-        // e never holds a leaf unless one is already held: a root that is a
-        // leaf is held at once (kNoRoot, an empty mesh, is kEntEnd)
-        e = in.root;
-        leaf = 0u;
-        if (ent_leaf(e)) {
-          leaf = e;
-          e = kEntEnd;
+          if (!kAny && inst_found) {
+            hit.inst = ii;
+            found = true;
+          }
+          if (st.overflow) return true;
         }
         return false;
       }
-      const uint32_t ni = te;
-      const float4 na = gld4(S.tnodes, 3 * ni), nb = gld4(S.tnodes, 3 * ni + 1);
-      if (kCount) ++c.node_tests;
-      // sign-selected slab (geometry/aabb.h:59-71): near/far by direction sign
-      const bool sx = d.x < 0.f, sy = d.y < 0.f, sz = d.z < 0.f;
-      const float bx0 = sx ? na.w : na.x, bx1 = sx ? na.x : na.w;
-      const float by0 = sy ? nb.x : na.y, by1 = sy ? na.y : nb.x;
-      const float bz0 = sz ? nb.y : na.z, bz1 = sz ? na.z : nb.y;
-      const float txn = (bx0 - o.x) * inv.x, txf = (bx1 - o.x) * inv.x;
-      const float tyn = (by0 - o.y) * inv.y, tyf = (by1 - o.y) * inv.y;
-      const float tzn = (bz0 - o.z) * inv.z, tzf = (bz1 - o.z) * inv.z;
-      const float lo_t = smax(smax(smax(tmin, txn), tyn), tzn);
-      const float hi_t = smin(smin(smin(tbest, txf), tyf), tzf);
-      if (!(lo_t <= hi_t)) continue;
-      const uint32_t child = __float_as_uint(nb.z), count = __float_as_uint(nb.w);
-      if (count > 0) {
-        for (uint32_t k = count; k-- > 0;) st.push(kInstTag | gld1(S.torder, child + k));
-      } else {
-        if (kCount) ++c.interior;
-        const float4 nc = gld4(S.tnodes, 3 * ni + 2);
-        const int ax = static_cast<int>(__float_as_uint(nc.z));
-        const float oa = comp(o, ax), ia = comp(inv, ax);
-        const float tl = (nc.x - oa) * ia;
-        const float tr = (nc.y - oa) * ia;
-        if (kAny) {
-          st.push(child);
-          st.push(child + 1);
-        } else if (tl < tr) {
-          st.push(child + 1);
-          st.push(child);
-        } else {
-          st.push(child);
-          st.push(child + 1);
-        }
+      blas_base = st.sp;
+      // e never holds a leaf unless one is already held: a root that is a
+      // leaf is held at once (kNoRoot, an empty mesh, is kEntEnd)
+      e = in.root;
+      leaf = 0u;
+      if (ent_leaf(e)) {
+        leaf = e;
+        e = kEntEnd;
       }
-      if (st.overflow) return true;
+      return false;
     }
-    return false;
+    const uint32_t ni = te;
+    const float4 na = gld4(S.tnodes, 3 * ni), nb = gld4(S.tnodes, 3 * ni + 1);
+    if (kCount) ++c.node_tests;
+    // sign-selected slab (geometry/aabb.h:59-71): near/far by direction sign
+    const bool sx = d.x < 0.f, sy = d.y < 0.f, sz = d.z < 0.f;
+    const float bx0 = sx ? na.w : na.x, bx1 = sx ? na.x : na.w;
+    const float by0 = sy ? nb.x : na.y, by1 = sy ? na.y : nb.x;
+    const float bz0 = sz ? nb.y : na.z, bz1 = sz ? na.z : nb.y;
+    const float txn = (bx0 - o.x) * inv.x, txf = (bx1 - o.x) * inv.x;
+    const float tyn = (by0 - o.y) * inv.y, tyf = (by1 - o.y) * inv.y;
+    const float tzn = (bz0 - o.z) * inv.z, tzf = (bz1 - o.z) * inv.z;
+    const float lo_t = smax(smax(smax(tmin, txn), tyn), tzn);
+    const float hi_t = smin(smin(smin(tbest, txf), tyf), tzf);
+    if (!(lo_t <= hi_t)) return false;
+    const uint32_t child = __float_as_uint(nb.z), count = __float_as_uint(nb.w);
+    if (count > 0) {
+      for (uint32_t k = count; k-- > 0;) st.push(kInstTag | gld1(S.torder, child + k));
+    } else {
+      if (kCount) ++c.interior;
+      const float4 nc = gld4(S.tnodes, 3 * ni + 2);
+      const int ax = static_cast<int>(__float_as_uint(nc.z));
+      const float oa = comp(o, ax), ia = comp(inv, ax);
+      const float tl = (nc.x - oa) * ia;
+      const float tr = (nc.y - oa) * ia;
+      if (kAny) {
+        st.push(child);
+        st.push(child + 1);
+      } else if (tl < tr) {
+        st.push(child + 1);
+        st.push(child);
+      } else {
+        st.push(child);
+        st.push(child + 1);
+      }
+    }
+    return st.overflow;
+  }
+
+  // The kind of step the lane needs next: 0 a TLAS step, 1 the held leaf,
+  // 2 an interior step. A finished BLAS walk (no entry left, no leaf held)
+  // first returns the lane to the TLAS.
+  __device__ __forceinline__ int next_kind() {
+    if (blas_base >= 0 && leaf == 0u && !ent_interior(e)) end_instance();
+    return blas_base < 0 ? 0 : (leaf != 0u ? 1 : 2);
   }
 
   // One interior step on e (an interior entry, no leaf held): pair-test its
@@ -658,42 +665,83 @@ struct Walker {
     return false;
   }
 
-  // One round; true once the ray is finished (closest: `found` / `hit`;
-  // any: `found` = occluded). Reached by the lanes that hold a ray; the
-  // phases are wave-wide over those lanes.
-  // Interior steps until no lane of the wave has an interior entry to walk.
-  template <bool kFast>
-  __device__ __forceinline__ void interior_phase(const DevScene& S, TravStack& st, TravCount& c) {
-    for (;;) {
-      const bool walk = leaf == 0u && ent_interior(e);
-      if (__ballot(walk) == 0ull) break;
-      if (kCount && first_active_lane()) ++c.int_iters;
-      if (walk) interior_step<kFast>(S, st, c);
-      if (st.overflow) break;
-    }
-  }
-
-  __device__ __forceinline__ bool round(const DevScene& S, TravStack& st, TravCount& c) {
-    if (tlas_steps(S, st, c)) return true;
-    // the hardware min/max slab test when every walking lane's ray allows it
-    // (a wave-uniform choice: one loop body per phase)
-    if (__ballot(leaf == 0u && ent_interior(e) && !fast) == 0ull) interior_phase<true>(S, st, c);
-    else interior_phase<false>(S, st, c);
-    if (st.overflow) {
-      end_instance();
-      return true;
-    }
-    for (;;) {
-      const bool has = leaf != 0u;
-      if (__ballot(has) == 0ull) break;
-      if (kCount && first_active_lane()) ++c.leaf_iters;
-      if (kCount && has) ++c.leaf_steps;
-      if (has && leaf_step(S, st, c)) return true;
-    }
-    if (e == kEntEnd && leaf == 0u) end_instance();
-    return false;
-  }
 };
+
+// One scheduled run of a wave's walks. Every lane holding a ray names the
+// kind of step it needs next (TLAS entry, held leaf, or interior node pair);
+// the kind most lanes need runs, on those lanes only, while the others wait:
+// interior and leaf steps repeat while their kind keeps the majority of the
+// wave's BLAS lanes, TLAS steps while at least half of the lanes that began
+// the run still need one. Each lane's own steps still run one at a time in
+// the reference's order, each seeing the t_best its earlier steps left:
+// only the interleaving of different lanes' steps is chosen here, so results
+// and work counts are the reference's. Scheduling by majority rather than
+// letting every lane finish a whole descent before any leaf is tested keeps
+// more lanes of each iteration busy (a wave's descents differ in length).
+// True on lanes whose ray finished (closest: `found` / `hit`; any: `found` =
+// occluded). Every lane of the wave must reach it.
+template <bool kAny, bool kCount, bool kPrims>
+__device__ __forceinline__ bool wave_step(Walker<kAny, kCount, kPrims>& w, const DevScene& S, TravStack& st,
+                                          TravCount& c, bool active) {
+  constexpr int kTlas = 0, kLeaf = 1, kInterior = 2, kWait = 3;
+  int k = active ? w.next_kind() : -1;
+  const int nt = __popcll(__ballot(k == kTlas));
+  int nl = __popcll(__ballot(k == kLeaf)), ni = __popcll(__ballot(k == kInterior));
+  bool done = false;
+  if (ni >= nl && ni >= nt) {
+    // the hardware min/max slab test when every walking lane's ray allows it
+    // (a wave-uniform choice)
+    const bool all_fast = __ballot(k == kInterior && !w.fast) == 0ull;
+    for (;;) {
+      if (kCount && first_active_lane()) ++c.int_iters;
+      if (k == kInterior) {
+        if (all_fast) w.template interior_step<true>(S, st, c);
+        else w.template interior_step<false>(S, st, c);
+        if (st.overflow) {
+          w.end_instance();
+          done = true;
+          k = -1;
+        } else {
+          k = w.leaf != 0u ? kLeaf : (ent_interior(w.e) ? kInterior : kWait);
+        }
+      }
+      ni = __popcll(__ballot(k == kInterior));
+      nl = __popcll(__ballot(k == kLeaf));
+      if (ni == 0 || ni < nl) break;
+    }
+  } else if (nl >= nt) {
+    for (;;) {
+      if (kCount && first_active_lane()) ++c.leaf_iters;
+      if (k == kLeaf) {
+        if (kCount) ++c.leaf_steps;
+        if (w.leaf_step(S, st, c)) {
+          done = true;
+          k = -1;
+        } else {
+          k = w.leaf != 0u ? kLeaf : (ent_interior(w.e) ? kInterior : kWait);
+        }
+      }
+      nl = __popcll(__ballot(k == kLeaf));
+      ni = __popcll(__ballot(k == kInterior));
+      if (nl == 0 || nl < ni) break;
+    }
+  } else {
+    for (;;) {
+      if (k == kTlas) {
+        if (w.tlas_step(S, st, c) || st.overflow) {
+          if (st.overflow) w.end_instance();
+          done = true;
+          k = -1;
+        } else if (w.blas_base >= 0) {
+          k = kWait;
+        }
+      }
+      const int n0 = __popcll(__ballot(k == kTlas));
+      if (n0 == 0 || 2 * n0 < nt) break;
+    }
+  }
+  return done;
+}
 
 // Whole walk of one ray per lane (the ray-batch API), with work counts.
 // Every lane of the wave must reach it.
@@ -704,11 +752,11 @@ __device__ __forceinline__ bool walk_scene(const DevScene& S, f3 o, f3 d, float 
   w.begin(S, o, d, tmin, tmax, st);
   bool done = !valid;
   for (uint32_t guard = 0; __ballot(!done) != 0ull; ++guard) {
-    if (guard >= kWaveRoundLimit) {
+    if (guard >= kWaveStepLimit) {
       w.stuck = true;
       break;
     }
-    if (!done) done = w.round(S, st, c);
+    if (wave_step(w, S, st, c, !done)) done = true;
   }
   if (w.stuck) st.overflow = true;
   hit = w.hit;

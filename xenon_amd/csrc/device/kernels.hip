@@ -116,7 +116,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
   bool exhausted = false;
   bool stuck = false;
   for (uint32_t guard = 0;; ++guard) {
-    if (guard >= kWaveRoundLimit) {
+    if (guard >= kWaveStepLimit) {
       stuck = true;
       break;
     }
@@ -159,16 +159,13 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
     }
     if (__ballot(my != kIdle) == 0ull) break;
     if (kCount && lane == 0) ++c.round_iters;
-    if (my != kIdle) {
-      if (kCount) ++c.round_lanes;
-      const DevScene& S = B.jobs[uniform(v.ch[pci].job)].S;
-      if (w.round(S, st, c)) {
-        overflow |= st.overflow;
-        stuck |= w.stuck;
-        finish(my, w);
-        ++rays;
-        my = kIdle;
-      }
+    if (kCount && my != kIdle) ++c.round_lanes;
+    const DevScene& S = B.jobs[uniform(v.ch[pci].job)].S;
+    if (wave_step(w, S, st, c, my != kIdle)) {
+      overflow |= st.overflow;
+      finish(my, w);
+      ++rays;
+      my = kIdle;
     }
   }
   if (__ballot(stuck) != 0ull && lane_id() == 0) atomicAdd(&B.counters[kWatchdog], 1u);
@@ -213,22 +210,77 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
 // shade: emission + MIS, area and sun NEE, Russian roulette, BSDF sampling
 // (cpu_tiled.cpp:30-216) for every entry of queue qi at depth `depth`.
 // Survivors and shadow rays are appended to the same chunk's segments.
+//
+// Each wave takes a window of kShadeWin consecutive 64-entry units and, when
+// the window lies in one chunk, orders its entries by shading class (miss,
+// Lambert, conductor, ...: the branch the entry takes below) before shading
+// them 64 at a time, so that most of its passes run one branch with all
+// lanes instead of every branch present with a few lanes each. Every
+// entry's shading depends only on its own path, so the order changes
+// neither its result nor where it is written (per-slot radiance terms;
+// appended rays carry their slot).
 // ---------------------------------------------------------------------------
+constexpr uint32_t kShadeWin = 4;
+
 template <bool kGuided>
 __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth) {
   __shared__ BatchView v;
+  __shared__ uint16_t perm_all[256 / 64][kShadeWin * 64];   // per wave: window positions in class order
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
   load_batch(v, B, counts, false);
   const uint32_t total = v.prefix[v.nch];
   const PathQueue q = B.q[qi];
   const PathQueue qn = B.q[1 - qi];
+  const uint32_t wv = threadIdx.x >> 6;
+  uint16_t* perm = perm_all[wv];
+  const uint32_t lane = static_cast<uint32_t>(lane_id());
+  const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-  for (uint32_t unit = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); unit < total; unit += waves) {
-    const int ci = uniform(find_chunk(v, unit));
+  for (uint32_t u0 = (blockIdx.x * (blockDim.x >> 6) + wv) * kShadeWin; u0 < total; u0 += waves * kShadeWin) {
+    const uint32_t nu = min(kShadeWin, total - u0);
+    const int ci0 = uniform(find_chunk(v, u0));
+    const bool sorted = uniform(find_chunk(v, u0 + nu - 1)) == ci0;
+    if (sorted) {
+      // counting sort of the window's entries by class: class, then unit,
+      // then lane order
+      const DevChunk& ck0 = v.ch[ci0];
+      const DevScene& S0 = B.jobs[uniform(ck0.job)].S;
+      uint32_t key[kShadeWin];
+#pragma unroll
+      for (uint32_t u = 0; u < kShadeWin; ++u) {
+        const uint32_t off = (u0 + u - v.prefix[ci0]) * 64u + lane;
+        key[u] = kShadeInvalid;
+        if (u < nu && off < v.cnt[ci0]) {
+          const int inst = __float_as_int(B.hit[static_cast<uint32_t>(ck0.base) + off].w);
+          key[u] = inst < 0 ? static_cast<uint32_t>(kShadeMiss) : S0.ikind[inst];
+        }
+      }
+      uint32_t base = 0;
+      for (uint32_t k = 0; k < kNumShadeClasses; ++k) {
+#pragma unroll
+        for (uint32_t u = 0; u < kShadeWin; ++u) {
+          const unsigned long long m = __ballot(key[u] == k);
+          if (key[u] == k) perm[base + static_cast<uint32_t>(__popcll(m & below))] = static_cast<uint16_t>(u * 64u + lane);
+          base += static_cast<uint32_t>(__popcll(m));
+        }
+      }
+      // the window's positions are read by other lanes of this wave
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    for (uint32_t p = 0; p < nu; ++p) {
+    uint32_t pu = p, pl = lane;
+    if (sorted) {
+      const uint32_t pos = perm[p * 64u + lane];
+      pu = pos >> 6;
+      pl = pos & 63u;
+    }
+    const int ci = sorted ? ci0 : uniform(find_chunk(v, u0 + p));
     const DevChunk& ck = v.ch[ci];
     const DevJob& J = B.jobs[uniform(ck.job)];
     const DevScene& S = J.S;
-    const uint32_t off = (unit - v.prefix[ci]) * 64u + static_cast<uint32_t>(lane_id());
+    const uint32_t off = (u0 + pu - v.prefix[ci]) * 64u + pl;
     const uint32_t i = static_cast<uint32_t>(ck.base) + off;
     bool want_a = false, want_s = false, want_n = false;
     f3 ao{}, ad{}, ac{}, so{}, sd{}, sc{};
@@ -468,6 +520,9 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
       qn.thr[in] = make_float4(nT.x, nT.y, nT.z, 0.f);
       qn.st[in] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), slot, nrec);
     }
+    }
+    // every lane has read its positions before the next window writes them
+    __builtin_amdgcn_wave_barrier();
   }
 }
 

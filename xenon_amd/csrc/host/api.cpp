@@ -330,6 +330,23 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
     for (int v = 0; v < me.num_vertices; ++v)
       vnrm[vert_base[m] + v] = flat[m] ? f4(0.f, 0.f, 0.f, 0.f) : f4(me.nx[v], me.ny[v], me.nz[v], 0.f);
   }
+  // shading class of each mesh: its materials' common class (see ShadeClass)
+  auto mat_class = [&](int mid) -> uint32_t {
+    const xnh_material& m = fs->materials[mid];
+    if (m.flags & XNH_MAT_DELTA) return kShadeDelta;
+    if (m.flags & XNH_MAT_SUBSURFACE) return kShadeSubsurface;
+    if (m.flags & XNH_MAT_TRANSMISSIVE) return kShadeRoughGlass;
+    if (m.flags & XNH_MAT_CONDUCTOR) return kShadeConductor;
+    return m.f0[0] == 0.f ? kShadeLambert : kShadeDielectric;
+  };
+  std::vector<uint32_t> mesh_class(nm, kShadeMixed);
+  for (int m = 0; m < nm; ++m) {
+    const xnh_mesh& me = fs->meshes[m];
+    uint32_t k = me.num_triangles > 0 ? mat_class(me.mat_ids[0]) : kShadeMixed;
+    for (int t = 1; t < me.num_triangles && k != kShadeMixed; ++t)
+      if (mat_class(me.mat_ids[t]) != k) k = kShadeMixed;
+    mesh_class[m] = k;
+  }
   // TLAS nodes with their child-order key
   const int ntn = fs->num_tlas_nodes;
   std::vector<float4> tnodes(3 * static_cast<size_t>(ntn));
@@ -355,6 +372,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   // per-instance light lookup tables: first light naming (instance id, triangle)
   std::vector<int32_t> lmap;
   std::vector<float4> inst(4 * static_cast<size_t>(fs->num_instances));
+  std::vector<uint32_t> ikind(std::max(fs->num_instances, 1), kShadeMixed);
   for (int i = 0; i < fs->num_instances; ++i) {
     const xnh_instance& in = fs->instances[i];
     if (torder.size() > 0 && torder[i] >= static_cast<uint32_t>(fs->num_instances))
@@ -369,6 +387,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
       continue;
     }
     if (in.mesh < 0 || in.mesh >= nm) return fail(c, XNH_E_INVALID, "instance mesh out of range");
+    ikind[i] = mesh_class[in.mesh];
     int32_t lm = -1;
     const int ntri = fs->meshes[in.mesh].num_triangles;
     for (int l = 0; l < fs->num_lights; ++l) {
@@ -429,7 +448,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   free_all(g->allocs);
   DevScene S{};
   float4 *p_bn, *p_tr, *p_tn, *p_in, *p_vn, *p_li, *p_ma, *p_su, *p_pr = nullptr, *p_pn = nullptr;
-  uint32_t *p_to, *p_po = nullptr, *p_bc;
+  uint32_t *p_to, *p_po = nullptr, *p_bc, *p_ik;
   int32_t *p_vi, *p_tm, *p_lm;
   int rc = dev_upload(c, g->allocs, &p_bn, bnodes);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_bc, bigcnt);
@@ -441,6 +460,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_vn, vnrm);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_tm, tmat);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_lm, lmap);
+  if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_ik, ikind);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_li, lights);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_ma, mats);
   if (rc == XNH_OK) rc = dev_upload(c, g->allocs, &p_su, suns);
@@ -467,6 +487,7 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   S.prims = p_pr;
   S.pnodes = p_pn;
   S.porder = p_po;
+  S.ikind = p_ik;
   S.num_pnodes = np > 0 ? npn : 0;
   S.num_tnodes = ntn;
   S.num_lights = fs->num_lights;
@@ -482,7 +503,8 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
   g->bytes = 0;
   for (const auto* v : {&bnodes, &tris, &tnodes, &inst, &vnrm, &lights, &mats, &suns, &prims, &pnodes})
     g->bytes += v->size() * 16;
-  g->bytes += (torder.size() + vidx.size() + tmat.size() + lmap.size() + porder.size() + bigcnt.size()) * 4;
+  g->bytes += (torder.size() + vidx.size() + tmat.size() + lmap.size() + porder.size() + bigcnt.size() +
+               ikind.size()) * 4;
   return XNH_OK;
 }
 
