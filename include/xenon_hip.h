@@ -344,6 +344,18 @@ typedef struct {           /* NoiseStats (math/image.h:11-31) */
  * device. Images of width or height <= 2 give the all-zero result, as there.
  * At most 2^26 pixels. */
 int xnh_image_noise_stats(xnh_ctx* ctx, const float* rgb, int32_t width, int32_t height, xnh_noise_stats* out);
+/* xnh_image_noise_stats on a framebuffer already in device memory, with no
+ * allocation: no counterpart in the reference. For callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). rgb_dev: W*H*3 floats in device memory,
+ * the layout above, inside one device allocation (checked). The scratch is
+ * one static 32 MiB array per GPU, so at most 2^22 pixels (2048 x 2048);
+ * larger images return XNH_E_INVALID. The device address of that array is
+ * looked up once per device by xnh_create. Calls share the array: calls
+ * that overlap on one GPU (from two contexts or threads) race. The results
+ * equal xnh_image_noise_stats's on the same image. */
+int xnh_image_noise_stats_noalloc(xnh_ctx* ctx, const float* rgb_dev, int32_t width, int32_t height,
+                                  xnh_noise_stats* out);
 /* The 8-bit pixels save_image passes to stbi_write_png (main.cpp:26-31):
  * (uint8)(255.99 * clamp(c, 0, 1)^(1/2.2)), W*H*3 bytes in the file's row
  * order (the writer flips the framebuffer vertically, so file row r is

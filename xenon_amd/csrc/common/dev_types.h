@@ -222,6 +222,11 @@ struct ImgState {
   uint32_t prefix[2];          // bits of the selected key found so far
   uint32_t hist[512];          // [target][digit] of the current pass
 };
+// The allocation-free NoiseStats entry point's static scratch: images of at
+// most 2^22 pixels (2048 x 2048), luminance and keys 16 MiB each plus the
+// per-block partials and the state, 256-byte aligned.
+constexpr int64_t kImgStaticMaxPixels = int64_t(1) << 22;
+constexpr size_t kImgStaticScratchBytes = 33605888;   // 32.05 MiB
 struct ImgScratch {
   float* lum;                  // W*H luminance
   uint32_t* key;               // W*H residual keys

@@ -18,6 +18,8 @@
 
 #include <stdint.h>
 
+#include <mutex>
+
 #include "launch.h"
 
 namespace xnh {
@@ -309,6 +311,64 @@ hipError_t launch_noise_stats(hipStream_t s, const float* img, int w, int h, Img
     hipLaunchKernelGGL(k_img_pick, dim3(1), dim3(64), 0, s, sc.st);
   }
   return hipGetLastError();
+}
+
+// ---- scratch of the allocation-free NoiseStats entry point ----------------
+// One static device array per GPU, carved into the ImgScratch fields for
+// images of at most kImgStaticMaxPixels. The launch sequence above writes
+// every scratch word it later reads within the same call (the partial sums
+// per block, the keys per pixel; k_img_final clears the histogram and the
+// radix-select state), so nothing needs clearing between calls. Calls that
+// overlap on one GPU share the array and race.
+namespace {
+constexpr size_t align256(size_t x) { return (x + 255u) & ~static_cast<size_t>(255u); }
+constexpr size_t kStOffLum = 0;
+constexpr size_t kStOffKey = align256(kStOffLum + sizeof(float) * kImgStaticMaxPixels);
+constexpr size_t kStOffPart = align256(kStOffKey + sizeof(uint32_t) * kImgStaticMaxPixels);
+constexpr size_t kStOffPmax = align256(kStOffPart + sizeof(double) * 2 * kImgMaxBlocks);
+constexpr size_t kStOffPcnt = align256(kStOffPmax + sizeof(float) * kImgMaxBlocks);
+constexpr size_t kStOffState = align256(kStOffPcnt + sizeof(uint32_t) * kImgMaxBlocks);
+constexpr size_t kStBytes = align256(kStOffState + sizeof(ImgState));
+static_assert(kStBytes == kImgStaticScratchBytes, "static scratch size");
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch[kMaxDevices] = {};   // the array's device address, per device
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kStBytes];
+
+hipError_t img_static_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return hipSuccess;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != device) return hipErrorInvalidDevice;   // the address is the current device's
+  void* p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_scratch[device] = static_cast<unsigned char*>(p);
+  return hipSuccess;
+}
+
+hipError_t img_static_scratch(int device, ImgScratch* sc) {
+  if (device < 0 || device >= kMaxDevices || !sc) return hipErrorInvalidDevice;
+  unsigned char* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    base = g_scratch[device];
+  }
+  if (!base) return hipErrorNotInitialized;
+  sc->lum = reinterpret_cast<float*>(base + kStOffLum);
+  sc->key = reinterpret_cast<uint32_t*>(base + kStOffKey);
+  sc->part = reinterpret_cast<double*>(base + kStOffPart);
+  sc->pmax = reinterpret_cast<float*>(base + kStOffPmax);
+  sc->pcnt = reinterpret_cast<uint32_t*>(base + kStOffPcnt);
+  sc->st = reinterpret_cast<ImgState*>(base + kStOffState);
+  return hipSuccess;
 }
 
 hipError_t launch_img_rgb8(hipStream_t s, const float* img, int w, int h, uint8_t* out) {

@@ -62,6 +62,11 @@ hipError_t launch_to_half(hipStream_t s, const float* src, uint16_t* dst, size_t
 hipError_t launch_img_from_accum(hipStream_t s, const float4* accum, float inv_spp, int w, int h, float* img);
 hipError_t launch_noise_stats(hipStream_t s, const float* img, int w, int h, ImgScratch sc);
 hipError_t launch_img_rgb8(hipStream_t s, const float* img, int w, int h, uint8_t* out);
+// The static scratch array of image.hip: its device address is looked up once
+// per device (with that device current) by img_static_scratch_init;
+// img_static_scratch carves it into an ImgScratch.
+hipError_t img_static_scratch_init(int device);
+hipError_t img_static_scratch(int device, ImgScratch* sc);
 
 // BLAS on the device (bvh.hip). build: n >= 2 triangles (vertex indices
 // checked by the caller) -> 2n-1 nodes in the reference's node format and n

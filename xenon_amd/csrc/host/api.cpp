@@ -76,6 +76,7 @@ struct GuideState {
 
 struct xnh_ctx {
   xnh_config cfg{};
+  int device = 0;                 // the device current at xnh_create
   GuideState guide;
   std::string err;
   hipStream_t stream = nullptr;
@@ -929,6 +930,16 @@ int xnh_create(const xnh_config* cfg, xnh_ctx** out) {
     delete c;
     return XNH_E_INVALID;
   }
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return XNH_E_NO_DEVICE;
+  }
+  // The allocation-free entry point's static scratch: its device address,
+  // once per device (xnh_image_noise_stats_noalloc).
+  if (img_static_scratch_init(c->device) != hipSuccess) {
+    delete c;
+    return XNH_E_HIP;
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev_begin) != hipSuccess || hipEventCreate(&c->ev_end) != hipSuccess) {
     xnh_destroy(c);
@@ -1172,6 +1183,36 @@ int xnh_image_noise_stats(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, xn
   return read_noise_stats(c, c->out.sc.st, w, h, out);
 }
 
+int xnh_image_noise_stats_noalloc(xnh_ctx* c, const float* rgb_dev, int32_t w, int32_t h, xnh_noise_stats* out) {
+  if (!c || !out || w < 0 || h < 0) return XNH_E_INVALID;
+  *out = xnh_noise_stats{};
+  if (w <= 2 || h <= 2) return XNH_OK;   // image.h:41-42
+  if (!rgb_dev) return fail(c, XNH_E_INVALID, "no image");
+  const int64_t npix = static_cast<int64_t>(w) * h;
+  if (npix > kImgStaticMaxPixels)
+    return fail(c, XNH_E_INVALID, "image larger than the static scratch (2^22 pixels)");
+  // The image must lie inside one device allocation: the kernels read
+  // 3 * w * h floats from rgb_dev unchecked.
+  hipDeviceptr_t base = nullptr;
+  size_t bytes = 0;
+  const hipError_t range = hipMemGetAddressRange(&base, &bytes, const_cast<float*>(rgb_dev));
+  if (range != hipSuccess) {
+    (void)hipGetLastError();   // not device memory: clear the error the lookup left
+    return fail(c, XNH_E_INVALID, "the image is not in device memory");
+  }
+  const uintptr_t b = reinterpret_cast<uintptr_t>(base), p = reinterpret_cast<uintptr_t>(rgb_dev);
+  if (p < b || p - b + sizeof(float) * 3 * static_cast<size_t>(npix) > bytes)
+    return fail(c, XNH_E_INVALID, "the image extends past the end of its device allocation");
+  ImgScratch sc{};
+  hipError_t e = img_static_scratch(c->device, &sc);
+  if (e == hipErrorNotInitialized) {   // set-up did not run on this device (xnh_create does it)
+    e = img_static_scratch_init(c->device);
+    if (e == hipSuccess) e = img_static_scratch(c->device, &sc);
+  }
+  if (e != hipSuccess) return fail(c, XNH_E_HIP, std::string("static scratch: ") + hipGetErrorString(e));
+  HIP_OR_FAIL(c, launch_noise_stats(c->stream, rgb_dev, w, h, sc));
+  return read_noise_stats(c, sc.st, w, h, out);
+}
 
 int xnh_image_rgb8(xnh_ctx* c, const float* rgb, int32_t w, int32_t h, uint8_t* out) {
   if (!c || !out || w < 0 || h < 0) return XNH_E_INVALID;

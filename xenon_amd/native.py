@@ -167,6 +167,7 @@ SIGNATURES = {
     "xnh_render": (C.c_int, [_vp, C.POINTER(Camera), C.c_int32, C.c_int32, _fp]),
     "xnh_read_accum": (C.c_int, [_vp, _fp, _ip]),
     "xnh_image_noise_stats": (C.c_int, [_vp, _fp, C.c_int32, C.c_int32, C.POINTER(NoiseStats)]),
+    "xnh_image_noise_stats_noalloc": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, C.POINTER(NoiseStats)]),
     "xnh_image_rgb8": (C.c_int, [_vp, _fp, C.c_int32, C.c_int32, _u8p]),
     "xnh_reset": (C.c_int, [_vp]),
     "xnh_spp": (C.c_int, [_vp]),
