@@ -13,8 +13,8 @@ same sample indices are rendered plain and guided; errors are per-pixel
 relative squared errors (channel mean of (x - r)^2 / (r^2 + 1e-2)) against a
 plain render of 16x the samples from disjoint sample indices. Guiding must
 lower both the median pixel error and the mean without the 0.1 % largest
-pixel errors by the factors below (measured values in the assertion
-messages). The full mean is set by single caustic outliers at these counts,
+pixel errors by the factors below (measured 1.29x and 1.23x; the assertion
+messages print the values of the run). The full mean is set by single caustic outliers at these counts,
 for both methods alike, and is reported only.
 
 (Collected last: a regression here must not hide the parity tests.)"""
@@ -26,8 +26,8 @@ pytestmark = pytest.mark.gpu
 SPP = 1024
 REF_SPP = 16 * SPP
 TRAIN_SPP = 2048
-MEDIAN_GAIN_MIN = 1.15
-TRIM_GAIN_MIN = 1.05
+MEDIAN_GAIN_MIN = 1.18   # measured 1.29x
+TRIM_GAIN_MIN = 1.10     # measured 1.23x
 
 
 def _errors(img, ref):
