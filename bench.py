@@ -86,17 +86,20 @@ def relmse(x, r):
     return float(np.mean((x[ok] - r[ok]) ** 2 / (r[ok] ** 2 + 1e-2)))
 
 
-def relmse_stats(x, r, ref_err=None):
+def relmse_stats(x, r, ref_err=None, mask=None):
     """relMSE of image x against reference r (mean over finite pixels of the
     channel-mean (x - r)^2 / (r^2 + 1e-2)), the same with the 0.1 % largest
     pixel errors left out (single-path outliers dominate the mean at short
     times), and the median pixel error. ref_err: per-pixel error the
-    reference's own noise adds (subtracted, see guided_comparison)."""
+    reference's own noise adds (subtracted, see guided_comparison); mask:
+    only these pixels."""
     import numpy as np
     e = (x - r) ** 2 / (r ** 2 + 1e-2)
     e = e.reshape(-1, 3).mean(1)
     if ref_err is not None:
         e = e - ref_err
+    if mask is not None:
+        e = e[mask]
     e = e[np.isfinite(e)]
     s = np.sort(e)
     keep = s[: max(1, int(len(s) * 0.999))]
@@ -168,20 +171,38 @@ def guided_comparison(seconds: float, width: int, height: int, ref_spp: int, fre
     ep, eg = relmse_stats(plain, ref, ref_err), relmse_stats(guided, ref, ref_err)
     log(f"guided: plain {sp} spp {ep}; guided {sg} spp {eg}")
     r.close()
+    # the caustic region: pixels the glass makes at least 1.5x brighter than
+    # a diffuse white sphere in its place does (4096 plain samples of that
+    # variant); the errors are split between it and the rest of the room
+    dd = scenes.caustic_room(width=width, height=height, freq=freq)
+    dd.materials[3] = scenes.Material(base=(0.75, 0.75, 0.75), roughness=1.0, ior=1.0)
+    hd = HipScene(dd)
+    with HipRenderer(dd.width, dd.height, dd.max_bounces, dd.min_bounces) as r2:
+        r2.upload_scene(hd)
+        r2.render(hd.camera(), 20_000_001, 4096, want_image=False)
+        a2, n2 = r2.read_accum()
+    lum = np.array([0.2126, 0.7152, 0.0722], np.float32)
+    caustic = (ref @ lum) > 1.5 * ((a2 / n2) @ lum) + 1e-3
 
     # runs stop at the first batch boundary past the budget, so their
     # wall-clock differs a little; relMSE x seconds (the usual efficiency)
     # corrects for that, since relMSE falls as 1/samples
-    def ratio(k):
-        return round(ep[k] * tp / (eg[k] * tg), 3) if eg[k] > 0 else None
+    def ratio(a, b, k):
+        return round(a[k] * tp / (b[k] * tg), 3) if b[k] > 0 else None
 
+    regions = {}
+    for name, m in (("caustic", caustic), ("rest", ~caustic)):
+        pa, ga = relmse_stats(plain, ref, ref_err, m), relmse_stats(guided, ref, ref_err, m)
+        regions[name] = {"pixels": int(m.sum()), "trim999_ratio_plain_over_guided": ratio(pa, ga, "relmse_trim999"),
+                         "median_ratio_plain_over_guided": ratio(pa, ga, "median")}
     return {"workload": f"caustic room ({desc.num_tris} tris), {desc.width}x{desc.height}, depth {desc.max_bounces}",
             "seconds": round(seconds, 2), "reference_spp": ref_spp,
             "plain": {"spp": sp, "seconds": round(tp, 2), **ep},
             "guided": {"spp": sg, "seconds": round(tg, 2), **eg},
-            "relmse_ratio_plain_over_guided": ratio("relmse"),
-            "trim999_ratio_plain_over_guided": ratio("relmse_trim999"),
-            "median_ratio_plain_over_guided": ratio("median")}
+            "relmse_ratio_plain_over_guided": ratio(ep, eg, "relmse"),
+            "trim999_ratio_plain_over_guided": ratio(ep, eg, "relmse_trim999"),
+            "median_ratio_plain_over_guided": ratio(ep, eg, "median"),
+            "regions": regions}
 
 
 class _StdoutToStderr:
@@ -465,6 +486,12 @@ def main():
                 "kernel": "k_closest (closest-hit traversal)",
                 "bytes_per_launch": round(cbytes / launches), "avg_launch_ms": round(tot["closest_ms"] / launches, 4)}
     sbytes = algorithmic_bytes(tot, "shadow")
+    # shading: path state (64 B) and hit (20 B) in, material record (64 B),
+    # radiance term (16 B) out per shaded entry; the continuation (64 B) of
+    # every closest-hit ray after the camera rays; one shadow record (48 B)
+    # per shadow ray
+    hbytes = (164.0 * tot["closest_rays"] + 64.0 * max(0, tot["closest_rays"] - wl.paths * args.steps)
+              + 48.0 * tot["shadow_rays"])
     out = {
         "metric": METRIC,
         "value": round(mpaths, 3),
@@ -486,6 +513,10 @@ def main():
         "shadow_roofline": {"achieved": round(sbytes / (tot["shadow_ms"] / 1e3) / 1e9, 2) if tot["shadow_ms"] else None,
                             "frac": round(sbytes / (tot["shadow_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
                             if tot["shadow_ms"] else None, "unit": "GB/s", "kernel": "k_shadow (any-hit traversal)"},
+        "shade_roofline": {"achieved": round(hbytes / (tot["shade_ms"] / 1e3) / 1e9, 2) if tot["shade_ms"] else None,
+                           "frac": round(hbytes / (tot["shade_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+                           if tot["shade_ms"] else None, "unit": "GB/s", "kernel": "k_shade (shading)"},
+        "rays": {"paths": wl.paths * args.steps, "closest": int(tot["closest_rays"]), "shadow": int(tot["shadow_rays"])},
         "nonfinite_paths": int(tot["nonfinite"]),
         "simd": simd_use(tot),
         "work_per_ray": {k: round(tot[k] / max(1, tot[r]), 3) for k, r in (

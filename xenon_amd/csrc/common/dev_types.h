@@ -105,6 +105,9 @@ enum ShadeClass : uint32_t {
   kNumShadeClasses
 };
 
+// float4 per material record (device/dbsdf.h DMat)
+constexpr int kMatVecs = 6;
+
 struct DevCamera {
   float eye[3], llc[3], horiz[3], vert[3];
 };

@@ -10,17 +10,22 @@
 // Transcendentals (cos, sin, pow, exp) are the device's and may differ from
 // glibc by an ulp; everything else rounds as the host does.
 #pragma once
+#include "../common/dev_types.h"
 #include "dmath.h"
 #include "xenon_hip.h"
 
 namespace xnh {
 
-// Material as four float4 in HBM:
+// Material as six float4 in HBM (kMatVecs):
 //   {base.rgb, metallic} {ior, alpha, alpha_x, alpha_y} {f0.rgb, subsurface}
-//   {subsurface.rgb, flags}
+//   {subsurface.rgb, flags} {ms_k1, ms_k2, ms_f.r, ms_f.g} {ms_f.b, 0, 0, 0}
+// The last two hold the parts of the multiple-scattering compensation
+// (bsdf.h:212-249) that depend on the material alone, computed at upload
+// with the same float operations in the same order: ms_k1 = 5 exp(-2.69 a),
+// ms_k2 = 1 + 22.7 a^1.5 and ms_f = favg^2 E_avg / max(1 - favg.r (1 - E_avg), 1e-4).
 struct DMat {
-  f3 base, f0, ss;
-  float metallic, ior, alpha, ax, ay, sss;
+  f3 base, f0, ss, ms_f;
+  float metallic, ior, alpha, ax, ay, sss, ms_k1, ms_k2;
   uint32_t flags;
   __device__ __forceinline__ bool conductor() const { return flags & XNH_MAT_CONDUCTOR; }
   __device__ __forceinline__ bool transmissive() const { return flags & XNH_MAT_TRANSMISSIVE; }
@@ -29,7 +34,8 @@ struct DMat {
 };
 
 __device__ __forceinline__ DMat load_mat(const float4* mats, int id) {
-  const float4 a = mats[4 * id], b = mats[4 * id + 1], c = mats[4 * id + 2], d = mats[4 * id + 3];
+  const float4* r = mats + kMatVecs * id;
+  const float4 a = r[0], b = r[1], c = r[2], d = r[3], e = r[4], g = r[5];
   DMat m;
   m.base = xyz(a);
   m.metallic = a.w;
@@ -41,6 +47,9 @@ __device__ __forceinline__ DMat load_mat(const float4* mats, int id) {
   m.sss = c.w;
   m.ss = xyz(d);
   m.flags = __float_as_uint(d.w);
+  m.ms_k1 = e.x;
+  m.ms_k2 = e.y;
+  m.ms_f = f3{e.z, e.w, g.x};
   return m;
 }
 
@@ -146,19 +155,16 @@ __device__ __forceinline__ float vndf_pdf(f3 wo, f3 h, float ax, float ay) {
   return D * g * smax(0.f, dot(wo, h)) / co;
 }
 
-__device__ __forceinline__ float ms_E(float c, float a) {
-  const float e = 1.0f - pow_r(1.0f - c, 5.0f * exp_r(-2.69f * a)) / (1.0f + 22.7f * pow_r(a, 1.5f));
+// bsdf.h:212-249 per evaluation: the directional albedo E(c) and the
+// compensation factor, with the material's constant parts precomputed.
+__device__ __forceinline__ float ms_E(float c, const DMat& m) {
+  const float e = 1.0f - pow_r(1.0f - c, m.ms_k1) / m.ms_k2;
   return sclamp(e, 0.f, 1.f);
 }
-__device__ __forceinline__ f3 ms_comp(f3 f0, float co, float ci, float a) {
-  const float eo = ms_E(fabsf(co), a), ei = ms_E(fabsf(ci), a);
-  const float ea = 1.0f / (1.0f + 0.3037f * a + 0.1218f * a * a);
-  const f3 favg = (splat(1.f) + f0 * 20.f) * (1.f / 21.f);
-  float den = 1.f - favg.x * (1.f - ea);
-  den = smax(den, 1e-4f);
-  const f3 fms = favg * favg * ea / splat(den);
+__device__ __forceinline__ f3 ms_comp(const DMat& m, float co, float ci) {
+  const float eo = ms_E(fabsf(co), m), ei = ms_E(fabsf(ci), m);
   const float c = (1.f - eo) * (1.f - ei);
-  return splat(1.f) + fms * c / splat(smax(eo * ei, 1e-4f));
+  return splat(1.f) + m.ms_f * c / splat(smax(eo * ei, 1e-4f));
 }
 
 // ---- lobes ------------------------------------------------------------------
@@ -182,7 +188,7 @@ __device__ __forceinline__ f3 refl_f(f3 wo, f3 wi, const DMat& m) {
   const float den = 4.f * wo.z * wi.z;
   if (den < 1e-8f) return splat(0.f);
   f3 r = F * D * G / den;
-  if (!m.transmissive()) r = r * ms_comp(m.f0, wo.z, wi.z, m.alpha);
+  if (!m.transmissive()) r = r * ms_comp(m, wo.z, wi.z);
   return r;
 }
 __device__ __forceinline__ float refl_pdf(f3 wo, f3 wi, const DMat& m) {

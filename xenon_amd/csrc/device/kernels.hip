@@ -112,7 +112,15 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
   Walker<kAny, kCount, kPrims> w;
   uint32_t pnext = 0, pend = 0;   // wave-uniform pool of offsets in chunk pci
   int pci = 0;
-  int held = -1;                  // fetched unit of another job, waiting for the wave to drain
+  // Units are taken from the cursor several at a time: one global atomic
+  // on one address serves only ~85 M operations per second on the whole
+  // device, so a fetch per 64-entry unit capped a launch at ~5 G rays/s.
+  // A fetch takes up to 16 units, fewer as the queue runs out (what is left
+  // per 4 waves of the grid, as this wave last saw it), so the launch
+  // does not end with a few waves walking long private batches.
+  const uint32_t gwaves = gridDim.x * (blockDim.x >> 6);
+  uint32_t seen = 0;              // the first unit of this wave's last fetch
+  uint32_t ucur = 0, uend = 0;    // fetched units not yet taken (wave-uniform)
   bool exhausted = false;
   bool stuck = false;
   for (uint32_t guard = 0;; ++guard) {
@@ -123,24 +131,30 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
     const unsigned long long act = __ballot(my != kIdle);
     if (act != ~0ull) {
       if (pnext >= pend) {
-        int unit = -1;
-        if (held >= 0) {
-          if (act == 0ull) {
-            unit = held;
-            held = -1;
+        if (ucur >= uend && !exhausted) {
+          const uint32_t left = total > seen ? total - seen : 0u;
+          const uint32_t per = max(1u, min(16u, left / (4u * gwaves)));
+          uint32_t u0 = 0;
+          if (lane == 0) u0 = atomicAdd(cursor, per);
+          u0 = static_cast<uint32_t>(uniform(static_cast<int>(__shfl(u0, 0))));
+          if (u0 >= total) {
+            exhausted = true;
+          } else {
+            ucur = u0;
+            uend = min(u0 + per, total);
+            seen = u0;
           }
-        } else if (!exhausted) {
-          uint32_t u = 0;
-          if (lane == 0) u = atomicAdd(cursor, 1u);
-          u = static_cast<uint32_t>(uniform(static_cast<int>(__shfl(u, 0))));
-          if (u >= total) exhausted = true;
-          else if (act != 0ull && v.ch[find_chunk(v, u)].job != v.ch[pci].job) held = static_cast<int>(u);
-          else unit = static_cast<int>(u);
         }
-        if (unit >= 0) {
-          pci = uniform(find_chunk(v, static_cast<uint32_t>(unit)));
-          pnext = (static_cast<uint32_t>(unit) - v.prefix[pci]) * 64u;
-          pend = min(pnext + 64u, v.cnt[pci]);
+        if (ucur < uend) {
+          // a unit of another job waits until the wave has drained (the
+          // rays in flight in a wave share one scene)
+          const int ci = uniform(find_chunk(v, ucur));
+          if (act == 0ull || v.ch[ci].job == v.ch[pci].job) {
+            pci = ci;
+            pnext = (ucur - v.prefix[ci]) * 64u;
+            pend = min(pnext + 64u, v.cnt[ci]);
+            ++ucur;
+          }
         }
       }
       if (pnext < pend) {

@@ -431,13 +431,32 @@ int flatten_upload(xnh_ctx* c, const xnh_flat_scene* fs, xnh_gpu_scene* g) {
     lights[4 * l + 2] = f4(p[2][0], p[2][1], p[2][2], L.emission[1]);
     lights[4 * l + 3] = f4(n[0], n[1], n[2], L.emission[2]);
   }
-  std::vector<float4> mats(4 * static_cast<size_t>(fs->num_materials));
+  std::vector<float4> mats(kMatVecs * static_cast<size_t>(fs->num_materials));
   for (int i = 0; i < fs->num_materials; ++i) {
     const xnh_material& m = fs->materials[i];
-    mats[4 * i] = f4(m.base_color[0], m.base_color[1], m.base_color[2], m.metallic);
-    mats[4 * i + 1] = f4(m.ior, m.alpha, m.alpha_x, m.alpha_y);
-    mats[4 * i + 2] = f4(m.f0[0], m.f0[1], m.f0[2], m.subsurface);
-    mats[4 * i + 3] = f4(m.subsurface_color[0], m.subsurface_color[1], m.subsurface_color[2], bits(m.flags));
+    float4* r = mats.data() + kMatVecs * i;
+    r[0] = f4(m.base_color[0], m.base_color[1], m.base_color[2], m.metallic);
+    r[1] = f4(m.ior, m.alpha, m.alpha_x, m.alpha_y);
+    r[2] = f4(m.f0[0], m.f0[1], m.f0[2], m.subsurface);
+    r[3] = f4(m.subsurface_color[0], m.subsurface_color[1], m.subsurface_color[2], bits(m.flags));
+    // the material-only parts of the multiple-scattering compensation
+    // (bsdf.h:212-249), in the device's float operation order; the two
+    // transcendentals evaluated in double and rounded once, as the device
+    // does for its own (device/dmath.h cos_r etc.)
+    const float a = m.alpha;
+    const float k1 = 5.0f * static_cast<float>(std::exp(static_cast<double>(-2.69f * a)));
+    const float k2 = 1.0f + 22.7f * static_cast<float>(std::pow(static_cast<double>(a), 1.5));
+    const float ea = 1.0f / (1.0f + 0.3037f * a + 0.1218f * a * a);
+    float fms[3];
+    const float fav0 = (1.0f + m.f0[0] * 20.f) * (1.f / 21.f);
+    float den = 1.f - fav0 * (1.f - ea);
+    den = (den < 1e-4f) ? 1e-4f : den;
+    for (int k = 0; k < 3; ++k) {
+      const float fav = (1.0f + m.f0[k] * 20.f) * (1.f / 21.f);
+      fms[k] = fav * fav * ea / den;
+    }
+    r[4] = f4(k1, k2, fms[0], fms[1]);
+    r[5] = f4(fms[2], 0.f, 0.f, 0.f);
   }
   std::vector<float4> suns(2 * static_cast<size_t>(fs->num_dir_lights));
   for (int i = 0; i < fs->num_dir_lights; ++i) {
