@@ -680,6 +680,30 @@ struct Walker {
 // more lanes of each iteration busy (a wave's descents differ in length).
 // True on lanes whose ray finished (closest: `found` / `hit`; any: `found` =
 // occluded). Every lane of the wave must reach it.
+// Interior steps of wave_step's lanes while interior steps keep the
+// majority of the wave's BLAS lanes.
+template <bool kFast, bool kAny, bool kCount, bool kPrims>
+__device__ __forceinline__ void interior_run(Walker<kAny, kCount, kPrims>& w, const DevScene& S, TravStack& st,
+                                             TravCount& c, int& k, int& ni, int& nl, bool& done) {
+  constexpr int kLeaf = 1, kInterior = 2, kWait = 3;
+  for (;;) {
+    if (kCount && first_active_lane()) ++c.int_iters;
+    if (k == kInterior) {
+      w.template interior_step<kFast>(S, st, c);
+      if (st.overflow) {
+        w.end_instance();
+        done = true;
+        k = -1;
+      } else {
+        k = w.leaf != 0u ? kLeaf : (ent_interior(w.e) ? kInterior : kWait);
+      }
+    }
+    ni = __popcll(__ballot(k == kInterior));
+    nl = __popcll(__ballot(k == kLeaf));
+    if (ni == 0 || ni < nl) break;
+  }
+}
+
 template <bool kAny, bool kCount, bool kPrims>
 __device__ __forceinline__ bool wave_step(Walker<kAny, kCount, kPrims>& w, const DevScene& S, TravStack& st,
                                           TravCount& c, bool active) {
@@ -690,25 +714,13 @@ __device__ __forceinline__ bool wave_step(Walker<kAny, kCount, kPrims>& w, const
   bool done = false;
   if (ni >= nl && ni >= nt) {
     // the hardware min/max slab test when every walking lane's ray allows it
-    // (a wave-uniform choice)
-    const bool all_fast = __ballot(k == kInterior && !w.fast) == 0ull;
-    for (;;) {
-      if (kCount && first_active_lane()) ++c.int_iters;
-      if (k == kInterior) {
-        if (all_fast) w.template interior_step<true>(S, st, c);
-        else w.template interior_step<false>(S, st, c);
-        if (st.overflow) {
-          w.end_instance();
-          done = true;
-          k = -1;
-        } else {
-          k = w.leaf != 0u ? kLeaf : (ent_interior(w.e) ? kInterior : kWait);
-        }
-      }
-      ni = __popcll(__ballot(k == kInterior));
-      nl = __popcll(__ballot(k == kLeaf));
-      if (ni == 0 || ni < nl) break;
-    }
+    // (a wave-uniform choice). The two runs are separate loops: with one
+    // loop and the choice inside it, the compiler computed the slab
+    // distances before the branch, and the min/max instructions then
+    // re-canonicalised each of them (a value from another block may be a
+    // signalling NaN), 12 more instructions per node pair.
+    if (__ballot(k == kInterior && !w.fast) == 0ull) interior_run<true>(w, S, st, c, k, ni, nl, done);
+    else interior_run<false>(w, S, st, c, k, ni, nl, done);
   } else if (nl >= nt) {
     for (;;) {
       if (kCount && first_active_lane()) ++c.leaf_iters;

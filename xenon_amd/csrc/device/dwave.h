@@ -31,6 +31,25 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {
   return base + rank;
 }
 
+// Two appends to one counter with one atomic: the `a` lanes take the first
+// popcount(a) places of the wave's block, the `b` lanes the places after
+// them. Must be reached by every lane of the wave.
+__device__ __forceinline__ void wave_append2(uint32_t* ctr, bool a, bool b, uint32_t& ia, uint32_t& ib) {
+  const unsigned long long ma = __ballot(a);
+  const unsigned long long mb = __ballot(b);
+  ia = ib = 0u;
+  if ((ma | mb) == 0ull) return;
+  const int lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t na = static_cast<uint32_t>(__popcll(ma));
+  const int leader = __ffsll(static_cast<long long>(ma | mb)) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(ctr, na + static_cast<uint32_t>(__popcll(mb)));
+  base = __shfl(base, leader);
+  ia = base + static_cast<uint32_t>(__popcll(ma & below));
+  ib = base + na + static_cast<uint32_t>(__popcll(mb & below));
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;

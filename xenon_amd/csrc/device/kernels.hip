@@ -514,8 +514,10 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
     }
     // convergent section: compacted appends into this chunk's segments
     const uint32_t sbase = 2u * static_cast<uint32_t>(ck.base);
-    const uint32_t ia = sbase + wave_append(&B.counters[kShCnt + ci], want_a);
-    const uint32_t is = sbase + wave_append(&B.counters[kShCnt + ci], want_s);
+    uint32_t ia, is;
+    wave_append2(&B.counters[kShCnt + ci], want_a, want_s, ia, is);
+    ia += sbase;
+    is += sbase;
     const uint32_t in = static_cast<uint32_t>(ck.base) +
                         wave_append(&B.counters[kCnt + (1 - qi) * kMaxChunks + ci], want_n);
     if (want_a) {
