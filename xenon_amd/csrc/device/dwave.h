@@ -17,6 +17,44 @@ constexpr float kShadowOff = 5e-4f;      // math/ray.h:14
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
 
+// The job table is written by the host before a batch and is read-only while
+// kernels run. Read through the constant address space at a wave-uniform
+// index, a job's fields are scalar loads (the scalar cache, counted by
+// lgkmcnt). As generic loads inside divergent code they were per-lane vector
+// loads, each a round trip before the array it names could be read, and each
+// wait for one also waited for the wave's pending stores (vmcnt counts both).
+// The returned references are generic aliases of the constant-space object;
+// once inlined, the compiler infers the address space of the loads through
+// them. (A copy by value was kept in scratch memory.) The host compilation
+// pass parses these bodies but never runs them; the cast exists only for the
+// device.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(4))) const DevJob ConstJob;
+__device__ __forceinline__ const DevJob& job_rec(const DevJob* jobs, int job) {
+  return reinterpret_cast<const ConstJob*>(reinterpret_cast<uintptr_t>(jobs))[job];
+}
+#else
+__device__ __forceinline__ const DevJob& job_rec(const DevJob* jobs, int job) { return jobs[job]; }
+#endif
+__device__ __forceinline__ const DevScene& job_scene(const DevJob* jobs, int job) { return job_rec(jobs, job).S; }
+
+// A batch's paths are laid out in 8x8 pixel tiles, so that the 64 primary
+// rays of a wave leave one compact patch of the image (a 64x1 row strip
+// spreads over eight times the width): the image is cut into bands of 8 rows
+// (the last band may be lower), a band into blocks of 8 columns (the last
+// block may be narrower), and a block is enumerated row by row. Maps the
+// tiled index t in [0, W*H) to the pixel index y*W + x (a bijection).
+__device__ __forceinline__ int tiled_pixel(int t, int W, int H) {
+  const int band = t / (8 * W);
+  const int r = t - band * 8 * W;
+  const int hb = min(8, H - 8 * band);
+  const int bx = r / (8 * hb);
+  const int wb = min(8, W - 8 * bx);
+  const int q = r - bx * 8 * hb;
+  const int yi = q / wb;
+  return (8 * band + yi) * W + 8 * bx + (q - yi * wb);
+}
+
 // Wave-aggregated append: one atomic per wave, ranks by ballot prefix.
 // Must be reached by every lane of the wave.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {

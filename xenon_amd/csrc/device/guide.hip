@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_encode(WaveBuffers B
     InferUnit u;
     if (!infer_unit(v, unit, depth, u)) continue;
     if (u.off >= u.lim) continue;
-    const DevScene& S = B.jobs[uniform(u.ck->job)].S;
+    const DevScene& S = job_scene(B.jobs, uniform(u.ck->job));
     bool want = false;
     f3 p{0.f, 0.f, 0.f}, n{0.f, 0.f, 1.f};
     const float4 hv = B.hit[u.i];

@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
     const DevJob& J = B.jobs[ck.job];
     const int local = gid - ck.base;
     const int s = local / ck.npix;
-    const int pixel = local - s * ck.npix;
+    const int pixel = tiled_pixel(local - s * ck.npix, J.W, J.H);
     const int spp = ck.spp_first + s;
     const uint64_t seed = static_cast<uint64_t>(pixel) + static_cast<uint64_t>(spp) * 0xdeadbeefULL;
     Rng rng = rng_for(seed);
@@ -160,7 +160,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
       if (pnext < pend) {
         const unsigned long long idle = ~act;
         const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
-        const DevScene& S = B.jobs[uniform(v.ch[pci].job)].S;
+        const DevScene& S = job_scene(B.jobs, uniform(v.ch[pci].job));
         if (my == kIdle && pnext + rank < pend) {
           my = seg * static_cast<uint32_t>(v.ch[pci].base) + pnext + rank;
           f3 o, d;
@@ -174,7 +174,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
     if (__ballot(my != kIdle) == 0ull) break;
     if (kCount && lane == 0) ++c.round_iters;
     if (kCount && my != kIdle) ++c.round_lanes;
-    const DevScene& S = B.jobs[uniform(v.ch[pci].job)].S;
+    const DevScene& S = job_scene(B.jobs, uniform(v.ch[pci].job));
     if (wave_step(w, S, st, c, my != kIdle)) {
       overflow |= st.overflow;
       finish(my, w);
@@ -258,16 +258,24 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
       // counting sort of the window's entries by class: class, then unit,
       // then lane order
       const DevChunk& ck0 = v.ch[ci0];
-      const DevScene& S0 = B.jobs[uniform(ck0.job)].S;
+      const DevScene& S0 = job_scene(B.jobs, uniform(ck0.job));
+      // all hit loads of the window, then all class loads: two round trips
+      // (an entry past the end reads the chunk's first, which exists)
+      const uint32_t off0 = (u0 - v.prefix[ci0]) * 64u + lane;
+      const uint32_t cnt0 = v.cnt[ci0];
+      const uint32_t hbase = static_cast<uint32_t>(ck0.base);
+      int inst[kShadeWin];
+#pragma unroll
+      for (uint32_t u = 0; u < kShadeWin; ++u) {
+        const uint32_t off = off0 + u * 64u;
+        inst[u] = __float_as_int(B.hit[hbase + (u < nu && off < cnt0 ? off : 0u)].w);
+      }
       uint32_t key[kShadeWin];
 #pragma unroll
       for (uint32_t u = 0; u < kShadeWin; ++u) {
-        const uint32_t off = (u0 + u - v.prefix[ci0]) * 64u + lane;
+        const uint32_t off = off0 + u * 64u;
         key[u] = kShadeInvalid;
-        if (u < nu && off < v.cnt[ci0]) {
-          const int inst = __float_as_int(B.hit[static_cast<uint32_t>(ck0.base) + off].w);
-          key[u] = inst < 0 ? static_cast<uint32_t>(kShadeMiss) : S0.ikind[inst];
-        }
+        if (u < nu && off < cnt0) key[u] = inst[u] < 0 ? static_cast<uint32_t>(kShadeMiss) : S0.ikind[inst[u]];
       }
       uint32_t base = 0;
       for (uint32_t k = 0; k < kNumShadeClasses; ++k) {
@@ -292,7 +300,7 @@ __global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth)
     }
     const int ci = sorted ? ci0 : uniform(find_chunk(v, u0 + p));
     const DevChunk& ck = v.ch[ci];
-    const DevJob& J = B.jobs[uniform(ck.job)];
+    const DevJob& J = job_rec(B.jobs, uniform(ck.job));
     const DevScene& S = J.S;
     const uint32_t off = (u0 + pu - v.prefix[ci]) * 64u + pl;
     const uint32_t i = static_cast<uint32_t>(ck.base) + off;
@@ -596,9 +604,10 @@ __global__ void __launch_bounds__(256) k_accumulate(WaveBuffers B) {
   for (uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x; gid < total; gid += gridDim.x * blockDim.x) {
     const int ci = find_chunk(v, gid);
     const DevChunk& ck = v.ch[ci];
-    const int p = static_cast<int>(gid - v.prefix[ci]);
-    float4* accum = B.jobs[ck.job].accum;
-    float4 a = accum[p];
+    const int p = static_cast<int>(gid - v.prefix[ci]);   // tiled pixel index
+    const DevJob& J = B.jobs[ck.job];
+    float4* accum = J.accum + tiled_pixel(p, J.W, J.H);
+    float4 a = *accum;
     for (int s = 0; s < ck.nsamples; ++s) {
       const int slot = ck.base + s * ck.npix + p;
       const float4 t = B.l_term[slot], n0 = B.l_nee[2 * slot], n1 = B.l_nee[2 * slot + 1];
@@ -608,7 +617,7 @@ __global__ void __launch_bounds__(256) k_accumulate(WaveBuffers B) {
       a.y = a.y + L.y;
       a.z = a.z + L.z;
     }
-    accum[p] = a;
+    *accum = a;
     paths += static_cast<unsigned long long>(ck.nsamples);
   }
   // one atomic per block and counter
