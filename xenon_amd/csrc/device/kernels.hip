@@ -234,10 +234,15 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
 // neither its result nor where it is written (per-slot radiance terms;
 // appended rays carry their slot).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kShadeWin = 4;
+// Units per shading window. Eight sort better than four and amortise the
+// window's class loads over more entries; at four waves per SIMD (128 VGPRs,
+// a few bytes spilled) the plain kernel beat three waves without the spill
+// (benchmark k_shade 225 -> 217 ms per two steps). The guided kernel keeps
+// its three waves.
+constexpr uint32_t kShadeWin = 8;
 
 template <bool kGuided>
-__global__ void __launch_bounds__(256) k_shade(WaveBuffers B, int qi, int depth) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuided ? 3 : 4))) k_shade(WaveBuffers B, int qi, int depth) {
   __shared__ BatchView v;
   __shared__ uint16_t perm_all[256 / 64][kShadeWin * 64];   // per wave: window positions in class order
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
