@@ -11,9 +11,11 @@
 //
 // Layout: grid params FP16 [level][slot][2]; MLP weights FP16 row-major
 // [in][out] per layer (48x64, 64x64, 64x64, 64x48). Matrix products use
-// v_mfma_f32_16x16x16f16 (FP16 operands, FP32 accumulation), one wave per
-// 64 samples; operand lane layout (checked on gfx950): lane l holds
-// A[l%16][4(l/16)+j], B[4(l/16)+j][l%16] and D[4(l/16)+j][l%16], j = 0..3.
+// v_mfma_f32_16x16x32_f16, and v_mfma_f32_16x16x16f16 for a last 16-wide
+// step (FP16 operands, FP32 accumulation), one wave per 64 samples; operand
+// lane layout (checked on gfx950): lane l holds D[4(l/16)+j][l%16], j =
+// 0..3, and its A and B values pair by position (16x16x16: A[l%16][4(l/16)+j],
+// B[4(l/16)+j][l%16]; 16x16x32: 8 values each, any order common to A and B).
 // Functions marked "wave" must be reached by all 64 lanes of a one-wave
 // block (they use __syncthreads as the LDS barrier).
 #pragma once
@@ -159,25 +161,52 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // ---- MFMA tiles -----------------------------------------------------------------
 
-// 16x16 tile of op(A) * op(B) over K (multiple of 16). A: 64 x K rows in LDS
+// 16x16 tile of op(A) * op(B) over K (a multiple of 16). A: 64 x K rows
 // (row-major, lda), rows r0..r0+15 (or its transpose when TA: element (m, k)
 // read at A[k * lda + m]); B: K x N row-major (ldb), cols c0..c0+15 (or its
-// transpose when TB: element (k, n) read at B[n * ldb + k]).
+// transpose when TB: element (k, n) read at B[n * ldb + k]). Steps of 32
+// inputs use gfx950's 16x16x32 form with lane l holding contraction index
+// k0 + 8 (l/16) + j, j = 0..7, in both operands, so an operand contiguous
+// along k (A rows, B transposed) is one 16-byte load per lane; a final
+// 16-input step (K % 32 = 16) uses the 16x16x16 form. Operands contiguous
+// along k must be 16-byte aligned with lda / ldb multiples of 8.
 template <bool TA, bool TB>
 __device__ __forceinline__ f4v mma_tile(const _Float16* A, int lda, int r0, const _Float16* B, int ldb, int c0,
                                         int K) {
+  typedef _Float16 h8 __attribute__((ext_vector_type(8)));
   const int lane = static_cast<int>(threadIdx.x & 63u);
-  const int li = lane & 15, lk = 4 * (lane >> 4);
+  const int li = lane & 15, g = lane >> 4;
   f4v acc = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  int k0 = 0;
+  for (; k0 + 32 <= K; k0 += 32) {
+    const int kb = k0 + 8 * g;
+    h8 a, b;
+    if (TA) {
+      for (int j = 0; j < 8; ++j) a[j] = A[(kb + j) * lda + r0 + li];
+    } else {
+      a = *reinterpret_cast<const h8*>(A + (r0 + li) * lda + kb);
+    }
+    if (TB) {
+      b = *reinterpret_cast<const h8*>(B + (c0 + li) * ldb + kb);
+    } else {
+      for (int j = 0; j < 8; ++j) b[j] = B[(kb + j) * ldb + c0 + li];
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, acc, 0, 0, 0);
+  }
+  // The 16-input steps accumulate apart and are added at the end: chaining
+  // one MFMA form's result into the other form's input C gave wrong sums on
+  // gfx950 (the compiler did not separate the two by enough wait states).
+  f4v acc16 = {0.f, 0.f, 0.f, 0.f};
+  for (; k0 < K; k0 += 16) {
+    const int kb = k0 + 4 * g;
     h4v a, b;
     for (int j = 0; j < 4; ++j) {
-      a[j] = TA ? A[(k0 + lk + j) * lda + r0 + li] : A[(r0 + li) * lda + k0 + lk + j];
-      b[j] = TB ? B[(c0 + li) * ldb + k0 + lk + j] : B[(k0 + lk + j) * ldb + c0 + li];
+      a[j] = TA ? A[(kb + j) * lda + r0 + li] : A[(r0 + li) * lda + kb + j];
+      b[j] = TB ? B[(c0 + li) * ldb + kb + j] : B[(kb + j) * ldb + c0 + li];
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, acc, 0, 0, 0);
+    acc16 = __builtin_amdgcn_mfma_f32_16x16x16f16(a, b, acc16, 0, 0, 0);
   }
-  return acc;
+  return acc + acc16;
 }
 
 // wave: Y (64 x N) = act(X (64 x K) W (K x N)); X, Y in LDS, W global.
@@ -198,26 +227,66 @@ __device__ __forceinline__ void dense_fwd(const _Float16* X, int K, const _Float
 
 // ---- register-resident forward pass -----------------------------------------------
 //
-// The MLP evaluated transposed, Y^T = W^T X^T: A = a 16x16 tile of W^T, B =
-// 16 features x 16 samples of X^T. The accumulator of one layer (lane l:
-// feature 16 mt + 4 (l/16) + j of sample l % 16) is then exactly the B
-// operand of the next layer, so hidden activations never leave registers.
-// The A tiles are stored pre-arranged (GuideNet::wt): layer L (K x N) holds
-// tile (mt, kt) at L's offset + (mt * K/16 + kt) * 256, lane l's four values
-// W[16 kt + 4 (l/16) + j][16 mt + l % 16] at + 4 l + j.
-__device__ __forceinline__ int tiled_index(int k, int n, int K) {
-  return ((n >> 4) * (K >> 4) + (k >> 4)) * 256 + ((((k & 15) >> 2) << 4) + (n & 15)) * 4 + (k & 3);
+// The MLP evaluated transposed, Y^T = W^T X^T, on gfx950's
+// v_mfma_f32_16x16x32_f16 (twice the work of the 16x16x16 form per
+// instruction): A = 16 outputs x 32 inputs of W^T, B = 32 features x 16
+// samples of X^T, lane l holding 8 values of each, D as for 16x16x16 (lane
+// l: output 4 (l/16) + j of sample l % 16, j = 0..3). The accumulators of
+// two adjacent output tiles of one layer, concatenated, are the B operand
+// of one 32-input step of the next layer: lane l then holds the 32-block's
+// features 4 (l/16) + j and 16 + 4 (l/16) + j, j = 0..3, and the weights
+// are tiled in that same order, so hidden activations never leave registers
+// (any consistent order of the contraction index gives the same product).
+// An input width that is not a multiple of 32 (the 48 encoded inputs) ends
+// with one 16x16x16 step. Weight tiles (GuideNet::wt), layer L (K x N):
+// 32-input tile (mt, kt) at L's offset + (mt * (K/32) + kt) * 512, lane l's
+// eight values at + 8 l + j: W[32 kt + f(l/16, j)][16 mt + l % 16] with
+// f(g, j) = 4 g + j (j < 4), 16 + 4 g + j - 4 (j >= 4); then, if K % 32 = 16,
+// the 16-input tiles mt at + (N/16) (K/32) 512 + 256 mt, lane l's four
+// values W[32 (K/32) + 4 (l/16) + j][16 mt + l % 16] at + 4 l + j.
+__device__ __forceinline__ int tiled_index(int k, int n, int K, int N) {
+  const int nkb = K >> 5, mt = n >> 4, li = n & 15;
+  if (k < (nkb << 5)) {
+    const int kt = k >> 5, kk = k & 31;
+    const int g = (kk & 15) >> 2, j = (kk & 3) + ((kk >> 4) << 2);
+    return ((mt * nkb + kt) * 64 + g * 16 + li) * 8 + j;
+  }
+  const int kk = k - (nkb << 5);
+  return (N >> 4) * nkb * 512 + (mt * 64 + (kk >> 2) * 16 + li) * 4 + (kk & 3);
 }
 
-template <int KT, int MT>
-__device__ __forceinline__ void dense_reg(const h4v* x, const h4v* w, h4v* y) {
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ h8v cat8(h4v lo, h4v hi) {
+  return h8v{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
+// One layer of MT output tiles over KT 16-feature input tiles x (KT even,
+// or KT = 3 with a 16-input step last); w: the layer's tiles in LDS as h4v
+// (16-byte aligned); relu: FP16 ReLU outputs, else the raw FP32 accumulators.
+template <int KT, int MT, bool kRelu>
+__device__ __forceinline__ void dense_reg(const h4v* x, const h4v* w, h4v* y, f4v* raw = nullptr) {
   const int lane = static_cast<int>(threadIdx.x & 63u);
+  constexpr int NKB = KT / 2;
+  const h8v* w8 = reinterpret_cast<const h8v*>(w);
   for (int mt = 0; mt < MT; ++mt) {
     f4v acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kt = 0; kt < KT; ++kt) acc = __builtin_amdgcn_mfma_f32_16x16x16f16(w[(mt * KT + kt) * 64 + lane], x[kt], acc, 0, 0, 0);
-    h4v r;
-    for (int j = 0; j < 4; ++j) r[j] = static_cast<_Float16>(fmaxf(acc[j], 0.f));
-    y[mt] = r;
+    for (int kb = 0; kb < NKB; ++kb)
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(w8[(mt * NKB + kb) * 64 + lane], cat8(x[2 * kb], x[2 * kb + 1]), acc, 0,
+                                                   0, 0);
+    if (KT & 1) {
+      // its own accumulator: one MFMA form never takes the other's result
+      // as its input C (see mma_tile)
+      const f4v zero = {0.f, 0.f, 0.f, 0.f};
+      acc += __builtin_amdgcn_mfma_f32_16x16x16f16(w[MT * NKB * 128 + mt * 64 + lane], x[KT - 1], zero, 0, 0, 0);
+    }
+    if (kRelu) {
+      h4v r;
+      for (int j = 0; j < 4; ++j) r[j] = static_cast<_Float16>(fmaxf(acc[j], 0.f));
+      y[mt] = r;
+    } else {
+      raw[mt] = acc;
+    }
   }
 }
 
@@ -250,15 +319,13 @@ __device__ __forceinline__ void guide_mlp_wave(_Float16* act, const h4v* wl) {
   wave_lds_sync();   // every row is read before outputs overwrite them
   for (int nt = 0; nt < 4; ++nt) {
     h4v a[4], b[4];
-    dense_reg<3, 4>(x[nt], wl + kW1 / 4, a);
-    dense_reg<4, 4>(a, wl + kW2 / 4, b);
-    dense_reg<4, 4>(b, wl + kW3 / 4, a);
-    for (int mt = 0; mt < kGOut / 16; ++mt) {
-      f4v acc = {0.f, 0.f, 0.f, 0.f};
-      for (int kt = 0; kt < 4; ++kt)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(wl[kW4 / 4 + (mt * 4 + kt) * 64 + lane], a[kt], acc, 0, 0, 0);
-      for (int j = 0; j < 4; ++j) act[(16 * nt + li) * kGIn + 16 * mt + 4 * g + j] = static_cast<_Float16>(acc[j]);
-    }
+    f4v out[kGOut / 16];
+    dense_reg<3, 4, true>(x[nt], wl + kW1 / 4, a);
+    dense_reg<4, 4, true>(a, wl + kW2 / 4, b);
+    dense_reg<4, 4, true>(b, wl + kW3 / 4, a);
+    dense_reg<4, kGOut / 16, false>(a, wl + kW4 / 4, nullptr, out);
+    for (int mt = 0; mt < kGOut / 16; ++mt)
+      for (int j = 0; j < 4; ++j) act[(16 * nt + li) * kGIn + 16 * mt + 4 * g + j] = static_cast<_Float16>(out[mt][j]);
   }
   wave_lds_sync();
 }

@@ -154,7 +154,7 @@ struct TrainLds {
 __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const float* pos, const float* nrm,
                                                    float* raw) {
   __shared__ _Float16 lds[64 * kGIn];
-  __shared__ h4v wl[kWTiles];
+  __shared__ __align__(16) h4v wl[kWTiles];
   stage_weights(net.wt, wl);
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(64) k_guide_eval(GuideNet net, int n, const fl
 __global__ void __launch_bounds__(64) k_guide_query(GuideNet net, int n, const float* pos, const float* nrm,
                                                     const float* u, float* dir, float* pdf) {
   __shared__ _Float16 lds[64 * kGIn];
-  __shared__ h4v wl[kWTiles];
+  __shared__ __align__(16) h4v wl[kWTiles];
   stage_weights(net.wt, wl);
   const int lane = glane();
   const int i = static_cast<int>(blockIdx.x) * 64 + lane;
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_mlp(WaveBuffers B, i
                                                                 const uint4* feat) {
   __shared__ BatchView v;
   __shared__ _Float16 act[kInferWaves][64 * kGIn];
-  __shared__ h4v wl[kWTiles];
+  __shared__ __align__(16) h4v wl[kWTiles];
   const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
   load_batch(v, B, counts, false);
   stage_weights(net.wt, wl);
@@ -415,7 +415,7 @@ __global__ void __launch_bounds__(256) k_tile_weights(const uint16_t* w, uint16_
     off = kW4; K = kGHid; N = kGOut;
   }
   const int r = i - off, k = r / N, c = r - k * N;
-  wt[off + tiled_index(k, c, K)] = w[i];
+  wt[off + tiled_index(k, c, K, N)] = w[i];
 }
 
 // Forward + loss + backward over n training samples; accumulates the
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(64) k_guide_train(GuideNet net, int n, const f
                                                     const float* dir, const float* lam, float lam_scale,
                                                     const uint32_t* n_dev, const float* lam_sum_dev, int slice,
                                                     int nslices, float* grid_g, float* w_part, float* loss_sum) {
-  __shared__ TrainLds s;
+  __shared__ __align__(16) TrainLds s;
   float* w_g = w_part + static_cast<size_t>(blockIdx.x) * kWTotal;   // this work-group's weight gradients
   int begin = 0;
   if (n_dev) {
