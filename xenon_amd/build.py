@@ -20,8 +20,13 @@ LIB = os.path.join(HERE, "lib", "libxenon_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
+# -amdgpu-atomic-optimizer-strategy=None: every global atomic here is
+# already issued by one lane with the wave's total; the optimizer's rewrite
+# of such an atomic reads its result right after issuing it, which
+# serialised the shading kernel's two appends of a pass.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-amdgpu-atomic-optimizer-strategy=None",
              "-Wno-unused-command-line-argument"]
 CXX_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-D__HIP_PLATFORM_AMD__",
              f"-I{ROCM}/include"]

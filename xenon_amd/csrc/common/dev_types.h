@@ -115,13 +115,33 @@ struct DevCamera {
 // Wavefront state. A render call is a list of jobs (scene, camera,
 // resolution, sample range, depth); the host cuts their samples into
 // batches of chunks, each chunk a contiguous sample range of one job, and
-// uploads all chunk tables once. Chunk c of a batch owns queue entries and
-// per-sample slots [base, base + cap) (cap = pixels * samples) in both path
-// queues and [2 base, 2 base + 2 cap) in the shadow queue: per-job queue
-// segments, so every 64-entry unit of work belongs to exactly one job and a
-// wave reads its job's scene through scalar loads. Slot order inside a chunk
-// is sample-major: slot = base + s * pixels + pixel.
+// uploads all chunk tables once. Chunk c of a batch owns the per-sample
+// slots [base, base + cap) (cap = pixels * samples; slot order is
+// sample-major: slot = base + s * pixels + pixel) and, in both path queues,
+// kQSeg segments of qseg entries from qbase (in the shadow queue twice
+// that, from 2 qbase): per-job queue segments, so every 64-entry unit of
+// work belongs to exactly one job and a wave reads its job's scene through
+// scalar loads. The shading kernel appends into the segment of its
+// work-group's group (block index mod kQSeg), each segment with its own
+// counter kCtrStride words from the next: one counter for all appends of a
+// chunk served too few atomics per second (one address is resolved past the
+// XCDs' L2 caches, one operation at a time). Ray generation writes depth 0
+// as segment 0, in slot order.
 constexpr int kMaxChunks = 64;
+constexpr int kQSeg = 8;
+constexpr int kCtrStride = 64;   // words between two segment counters (256 B)
+// Entries of a segment beyond cap / kQSeg. The shading kernel's grid holds a
+// multiple of 32 waves, and window k (eight 64-entry units) goes to wave
+// k mod waves, whose group is (wave / 4) mod kQSeg: of n consecutive
+// windows a group takes at most n / 8 + 4, so its outputs for a chunk of
+// cnt <= cap input entries (at most one continuation, two shadow rays per
+// entry) stay below cap / 8 + 64 + 4.25 * 512 per segment.
+constexpr int kSegSlack = 2304;
+// Segment size of a chunk of cap paths (a multiple of 64).
+constexpr int64_t qseg_for(int64_t cap) { return ((cap + kQSeg - 1) / kQSeg + kSegSlack + 63) / 64 * 64; }
+// Queue entries beyond the slots that a batch of up to kMaxChunks chunks
+// needs for its segments.
+constexpr int64_t kQueueSlack = int64_t(kMaxChunks) * kQSeg * (kSegSlack + 128);
 
 struct DevJob {
   DevScene S;
@@ -133,7 +153,8 @@ struct DevJob {
 
 struct DevChunk {
   int32_t job, spp_first, nsamples, npix;
-  int32_t base, cap, pad0, pad1;
+  int32_t base, cap;
+  int32_t qbase, qseg;   // the chunk's path-queue segments: kQSeg of qseg entries from qbase
 };
 
 struct PathQueue {
@@ -158,7 +179,8 @@ struct WaveBuffers {
   uint32_t* counters;       // see Ctr
   unsigned long long* stats;  // see Stat
   uint32_t* spill;  // traversal stack entries beyond the LDS part
-  int32_t capacity;
+  int32_t capacity;   // slots (paths) of a batch
+  int32_t qcap;       // entries of each path-queue array (capacity + kQueueSlack); the shadow queue holds twice
   // path guiding (null / 0 when off)
   float4* mix;      // [kMixVecs][mix_stride] mixture per queue entry (k_guide_infer)
   uint32_t mix_stride;  // queue entries a guided batch uses at most (<= capacity): the stride of mix and of
@@ -176,12 +198,17 @@ enum Ctr : int {
   kOverflow = 1,                         // lanes whose stack exceeded kMaxStack
   kRecCnt = 2,                           // guiding training records of the batch
   kWatchdog = 3,                         // waves that hit an iteration bound (see dtrace.h)
-  kCnt = 16,                             // [queue][chunk] live paths
-  kShCnt = kCnt + 2 * kMaxChunks,        // [chunk] shadow rays
-  kCursor = kShCnt + kMaxChunks,         // per-launch work cursors of a batch
+  kSegOverflow = 4,                      // appends past a queue segment's end (dropped; never expected)
+  kCnt = 16,                                        // [queue][chunk][segment] live paths (stride kCtrStride)
+  kShCnt = kCnt + 2 * kMaxChunks * kQSeg * kCtrStride,  // [chunk][segment] shadow rays
+  kCursor = kShCnt + kMaxChunks * kQSeg * kCtrStride,   // per-launch work cursors of a batch
   kNumCursors = 2 * 64 + 4,
   kNumCtr = kCursor + kNumCursors
 };
+
+// Counter of segment x of chunk ci: path queue q, shadow queue.
+constexpr int ctr_path(int q, int ci, int x) { return kCnt + ((q * kMaxChunks + ci) * kQSeg + x) * kCtrStride; }
+constexpr int ctr_shadow(int ci, int x) { return kShCnt + (ci * kQSeg + x) * kCtrStride; }
 
 enum Stat : int {
   kStPaths = 0,

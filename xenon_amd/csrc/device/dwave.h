@@ -88,6 +88,37 @@ __device__ __forceinline__ void wave_append2(uint32_t* ctr, bool a, bool b, uint
   ib = base + na + static_cast<uint32_t>(__popcll(mb & below));
 }
 
+// The shading kernel's appends of one pass: shadow records (the `a` lanes,
+// then the `b` lanes) to ctr_s and continuations (`c` lanes) to ctr_c. The
+// leader issues both atomics before either result is read, so their round
+// trips overlap (one after the other, the second atomic's wait had cost as
+// much as the rest of the pass). Must be reached by every lane of the wave.
+__device__ __forceinline__ void wave_append_pass(uint32_t* ctr_s, bool a, bool b, uint32_t& ia, uint32_t& ib,
+                                                 uint32_t* ctr_c, bool c, uint32_t& ic) {
+  const unsigned long long ma = __ballot(a), mb = __ballot(b), mc = __ballot(c);
+  ia = ib = ic = 0u;
+  if ((ma | mb | mc) == 0ull) return;
+  const int lane = lane_id();
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const uint32_t na = static_cast<uint32_t>(__popcll(ma));
+  const uint32_t ns = na + static_cast<uint32_t>(__popcll(mb)), nc = static_cast<uint32_t>(__popcll(mc));
+  const int leader = __ffsll(static_cast<long long>(ma | mb | mc)) - 1;
+  uint32_t bs = 0u, bc = 0u;
+  if (lane == leader) {
+    if (ns) bs = atomicAdd(ctr_s, ns);
+    if (nc) bc = atomicAdd(ctr_c, nc);
+  }
+  // one broadcast of both results: the wait for either comes after both
+  // atomics are issued
+  const unsigned long long both =
+      __shfl(static_cast<unsigned long long>(bc) << 32 | static_cast<unsigned long long>(bs), leader);
+  bs = static_cast<uint32_t>(both);
+  bc = static_cast<uint32_t>(both >> 32);
+  ia = bs + static_cast<uint32_t>(__popcll(ma & below));
+  ib = bs + na + static_cast<uint32_t>(__popcll(mb & below));
+  ic = bc + static_cast<uint32_t>(__popcll(mc & below));
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
@@ -270,34 +301,49 @@ __device__ __forceinline__ f3 sky_eval(const DevSky& k, f3 d) {
 
 // ---- batch bookkeeping ------------------------------------------------------
 
-// Per-block copy of the current batch's chunk table with prefix sums of
-// 64-entry work units (traversal / shading) or of pixels (accumulation).
+// Per-block copy of the current batch's chunk table with the entry count of
+// every queue segment and prefix sums of 64-entry work units over chunks
+// (traversal / shading) or of pixels (accumulation).
 struct BatchView {
   int batch, nch;
   DevChunk ch[kMaxChunks];
-  uint32_t cnt[kMaxChunks];
+  uint32_t cnt[kMaxChunks][kQSeg];
   uint32_t prefix[kMaxChunks + 1];
 };
 
-// Loads the batch and builds prefix[] from per-chunk counts (counts may be
-// null: prefix over pixels). Block-wide; ends with a barrier.
+// Loads the batch and builds prefix[] from the segment counts. counts: the
+// queue's counter of chunk 0, segment 0, as ctr_path / ctr_shadow give it,
+// with mul 1 (path queue) or 2 (shadow queue); null: prefix over pixels.
+// A count is clamped to its segment (segment 0 of a path queue may hold the
+// chunk's whole region: depth 0), so reads stay inside the chunk's region
+// even if an append ever overflowed a segment. Block-wide; ends with a
+// barrier.
 __device__ __forceinline__ void load_batch(BatchView& v, const WaveBuffers& B, const uint32_t* counts,
-                                           bool pixels) {
+                                           bool pixels, uint32_t mul = 1u) {
   if (threadIdx.x == 0) {
     v.batch = static_cast<int>(B.counters[kBatch]);
     v.nch = B.nchunks[v.batch];
   }
   __syncthreads();
-  if (static_cast<int>(threadIdx.x) < v.nch) {
-    v.ch[threadIdx.x] = B.chunks[v.batch * kMaxChunks + threadIdx.x];
-    v.cnt[threadIdx.x] = counts ? counts[threadIdx.x] : 0u;
-  }
+  if (static_cast<int>(threadIdx.x) < v.nch) v.ch[threadIdx.x] = B.chunks[v.batch * kMaxChunks + threadIdx.x];
+  __syncthreads();
+  if (counts)
+    for (int t = static_cast<int>(threadIdx.x); t < v.nch * kQSeg; t += static_cast<int>(blockDim.x)) {
+      const int ci = t / kQSeg, x = t % kQSeg;
+      const uint32_t qseg = static_cast<uint32_t>(v.ch[ci].qseg);
+      const uint32_t room = mul * ((mul == 1u && x == 0) ? static_cast<uint32_t>(kQSeg) * qseg : qseg);
+      v.cnt[ci][x] = min(counts[t * kCtrStride], room);
+    }
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t acc = 0;
     v.prefix[0] = 0;
     for (int c = 0; c < v.nch; ++c) {
-      acc += pixels ? static_cast<uint32_t>(v.ch[c].npix) : ((v.cnt[c] + 63u) >> 6);
+      if (pixels) {
+        acc += static_cast<uint32_t>(v.ch[c].npix);
+      } else {
+        for (int x = 0; x < kQSeg; ++x) acc += (v.cnt[c][x] + 63u) >> 6;
+      }
       v.prefix[c + 1] = acc;
     }
   }
@@ -313,6 +359,25 @@ __device__ __forceinline__ int find_chunk(const BatchView& v, uint32_t u) {
     else hi = mid;
   }
   return lo;
+}
+
+// Unit uc of chunk ci's units (segment by segment): returns the queue
+// position of its segment's first entry (mul = 1 path queue, 2 shadow
+// queue) and sets its first entry's offset in that segment and the
+// segment's entry count.
+__device__ __forceinline__ uint32_t unit_segment(const BatchView& v, int ci, uint32_t uc, uint32_t mul, uint32_t& first,
+                                                 uint32_t& lim) {
+  uint32_t acc = 0;
+  int x = 0;
+  for (; x < kQSeg - 1; ++x) {
+    const uint32_t n = (v.cnt[ci][x] + 63u) >> 6;
+    if (uc < acc + n) break;
+    acc += n;
+  }
+  first = (uc - acc) * 64u;
+  lim = v.cnt[ci][x];
+  const DevChunk& ck = v.ch[ci];
+  return mul * (static_cast<uint32_t>(ck.qbase) + static_cast<uint32_t>(x) * static_cast<uint32_t>(ck.qseg));
 }
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }

@@ -230,18 +230,21 @@ struct InferUnit {
 __device__ __forceinline__ bool infer_unit(const BatchView& v, uint32_t unit, int depth, InferUnit& u) {
   const int ci = uniform(find_chunk(v, unit));
   u.ck = &v.ch[ci];
-  const uint32_t first = (unit - v.prefix[ci]) * 64u;
-  u.lim = depth == 0 ? min(v.cnt[ci], static_cast<uint32_t>(u.ck->npix)) : v.cnt[ci];
+  uint32_t first, lim;
+  const uint32_t sbase = unit_segment(v, ci, unit - v.prefix[ci], 1u, first, lim);
+  // depth 0: segment 0 holds the chunk's paths in slot order, the pixels'
+  // first samples first
+  u.lim = depth == 0 ? min(lim, static_cast<uint32_t>(u.ck->npix)) : lim;
   if (first >= u.lim) return false;   // uniform: a later sample of every pixel in the unit
   u.off = first + static_cast<uint32_t>(glane());
-  u.i = static_cast<uint32_t>(u.ck->base) + u.off;
+  u.i = sbase + u.off;
   return true;
 }
 
 __global__ void __launch_bounds__(64 * kInferWaves) k_guide_encode(WaveBuffers B, int qi, int depth, GuideNet net,
                                                                    uint4* feat) {
   __shared__ BatchView v;
-  const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
+  const uint32_t* counts = B.counters + ctr_path(qi, 0, 0);
   load_batch(v, B, counts, false);
   const uint32_t total = v.prefix[v.nch];
   const PathQueue q = B.q[qi];
@@ -281,7 +284,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_mlp(WaveBuffers B, i
   __shared__ BatchView v;
   __shared__ _Float16 act[kInferWaves][64 * kGIn];
   __shared__ __align__(16) h4v wl[kWTiles];
-  const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
+  const uint32_t* counts = B.counters + ctr_path(qi, 0, 0);
   load_batch(v, B, counts, false);
   stage_weights(net.wt, wl);
   const uint32_t total = v.prefix[v.nch];

@@ -43,11 +43,13 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
   if (static_cast<int>(threadIdx.x) < v.nch) v.ch[threadIdx.x] = B.chunks[v.batch * kMaxChunks + threadIdx.x];
   __syncthreads();
   const int tid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-  if (tid < v.nch) {
-    const DevChunk& ck = v.ch[tid];
-    B.counters[kCnt + tid] = B.jobs[ck.job].maxb > 0 ? static_cast<uint32_t>(ck.cap) : 0u;
-    B.counters[kCnt + kMaxChunks + tid] = 0u;
-    B.counters[kShCnt + tid] = 0u;
+  if (tid < v.nch * kQSeg) {
+    // depth 0: the chunk's paths as its segment 0, in slot order
+    const int ci = tid / kQSeg, x = tid % kQSeg;
+    const DevChunk& ck = v.ch[ci];
+    B.counters[ctr_path(0, ci, x)] = (x == 0 && B.jobs[ck.job].maxb > 0) ? static_cast<uint32_t>(ck.cap) : 0u;
+    B.counters[ctr_path(1, ci, x)] = 0u;
+    B.counters[ctr_shadow(ci, x)] = 0u;
   }
   if (tid < kNumCursors) B.counters[kCursor + tid] = 0u;
   if (tid == 0) B.counters[kRecCnt] = 0u;
@@ -79,11 +81,12 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
     const f3 hz{J.cam.horiz[0], J.cam.horiz[1], J.cam.horiz[2]};
     const f3 vt{J.cam.vert[0], J.cam.vert[1], J.cam.vert[2]};
     const f3 d = normalize(llc + u * hz + w * vt - eye);
-    q.ray0[gid] = make_float4(eye.x, eye.y, eye.z, d.x);
-    q.ray1[gid] = make_float4(d.y, d.z, 0.f, __uint_as_float(1u));
-    q.thr[gid] = make_float4(1.f, 1.f, 1.f, 0.f);
-    q.st[gid] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), static_cast<uint32_t>(gid),
-                           kNoRecord);
+    const int e = ck.qbase + local;   // queue entry (segment 0)
+    q.ray0[e] = make_float4(eye.x, eye.y, eye.z, d.x);
+    q.ray1[e] = make_float4(d.y, d.z, 0.f, __uint_as_float(1u));
+    q.thr[e] = make_float4(1.f, 1.f, 1.f, 0.f);
+    q.st[e] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), static_cast<uint32_t>(gid),
+                         kNoRecord);
     B.l_term[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
     B.l_nee[2 * gid] = make_float4(0.f, 0.f, 0.f, 0.f);
     B.l_nee[2 * gid + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -110,7 +113,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
   const unsigned long long below = (1ull << lane) - 1ull;
   uint32_t my = kIdle;
   Walker<kAny, kCount, kPrims> w;
-  uint32_t pnext = 0, pend = 0;   // wave-uniform pool of offsets in chunk pci
+  uint32_t pbase = 0, pnext = 0, pend = 0;   // wave-uniform pool: entries pbase + [pnext, pend) of chunk pci
   int pci = 0;
   // Units are taken from the cursor several at a time: one global atomic
   // on one address serves only ~85 M operations per second on the whole
@@ -151,8 +154,10 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
           const int ci = uniform(find_chunk(v, ucur));
           if (act == 0ull || v.ch[ci].job == v.ch[pci].job) {
             pci = ci;
-            pnext = (ucur - v.prefix[ci]) * 64u;
-            pend = min(pnext + 64u, v.cnt[ci]);
+            uint32_t first, lim;
+            pbase = unit_segment(v, ci, ucur - v.prefix[ci], seg, first, lim);
+            pnext = first;
+            pend = min(first + 64u, lim);
             ++ucur;
           }
         }
@@ -162,7 +167,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
         const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
         const DevScene& S = job_scene(B.jobs, uniform(v.ch[pci].job));
         if (my == kIdle && pnext + rank < pend) {
-          my = seg * static_cast<uint32_t>(v.ch[pci].base) + pnext + rank;
+          my = pbase + pnext + rank;
           f3 o, d;
           float tmin, tmax;
           load(my, o, d, tmin, tmax);
@@ -192,12 +197,13 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
 template <bool kCount, bool kPrims>
 __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int cursor, int lds_depth) {
   __shared__ BatchView v;
-  const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
+  const uint32_t* counts = B.counters + ctr_path(qi, 0, 0);
   load_batch(v, B, counts, false);
-  if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) < v.nch) {
-    B.counters[kCnt + (1 - qi) * kMaxChunks + threadIdx.x] = 0u;
-    B.counters[kShCnt + threadIdx.x] = 0u;
-  }
+  if (blockIdx.x == 0)
+    for (int t = static_cast<int>(threadIdx.x); t < v.nch * kQSeg; t += static_cast<int>(blockDim.x)) {
+      B.counters[ctr_path(1 - qi, t / kQSeg, t % kQSeg)] = 0u;
+      B.counters[ctr_shadow(t / kQSeg, t % kQSeg)] = 0u;
+    }
   const PathQueue q = B.q[qi];
   TravStack st = make_stack(B.spill, lds_depth);
   TravCount c{0, 0, 0, 0};
@@ -245,13 +251,15 @@ template <bool kGuided>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuided ? 3 : 4))) k_shade(WaveBuffers B, int qi, int depth) {
   __shared__ BatchView v;
   __shared__ uint16_t perm_all[256 / 64][kShadeWin * 64];   // per wave: window positions in class order
-  const uint32_t* counts = B.counters + kCnt + qi * kMaxChunks;
+  __shared__ uint2 uinfo_all[256 / 64][kShadeWin];           // per wave: units' first queue entry, entry count
+  const uint32_t* counts = B.counters + ctr_path(qi, 0, 0);
   load_batch(v, B, counts, false);
   const uint32_t total = v.prefix[v.nch];
   const PathQueue q = B.q[qi];
   const PathQueue qn = B.q[1 - qi];
   const uint32_t wv = threadIdx.x >> 6;
   uint16_t* perm = perm_all[wv];
+  uint2* uinfo = uinfo_all[wv];
   const uint32_t lane = static_cast<uint32_t>(lane_id());
   const unsigned long long below = (1ull << lane) - 1ull;
   const uint32_t waves = gridDim.x * (blockDim.x >> 6);
@@ -259,28 +267,37 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
     const uint32_t nu = min(kShadeWin, total - u0);
     const int ci0 = uniform(find_chunk(v, u0));
     const bool sorted = uniform(find_chunk(v, u0 + nu - 1)) == ci0;
+    // every unit's first queue entry and entry count (wave-uniform; lane u
+    // keeps unit u's in LDS for the passes)
+    uint32_t upos[kShadeWin], ucnt[kShadeWin];
+#pragma unroll
+    for (uint32_t u = 0; u < kShadeWin; ++u) {
+      upos[u] = 0u;
+      ucnt[u] = 0u;
+      if (u < nu) {
+        const int cu = sorted ? ci0 : uniform(find_chunk(v, u0 + u));
+        uint32_t first, lim;
+        upos[u] = unit_segment(v, cu, u0 + u - v.prefix[cu], 1u, first, lim) + first;
+        ucnt[u] = lim > first ? min(64u, lim - first) : 0u;
+      }
+      if (lane == u) uinfo[u] = make_uint2(upos[u], ucnt[u]);
+    }
     if (sorted) {
       // counting sort of the window's entries by class: class, then unit,
       // then lane order
       const DevChunk& ck0 = v.ch[ci0];
       const DevScene& S0 = job_scene(B.jobs, uniform(ck0.job));
       // all hit loads of the window, then all class loads: two round trips
-      // (an entry past the end reads the chunk's first, which exists)
-      const uint32_t off0 = (u0 - v.prefix[ci0]) * 64u + lane;
-      const uint32_t cnt0 = v.cnt[ci0];
-      const uint32_t hbase = static_cast<uint32_t>(ck0.base);
+      // (an entry past the end reads the window's first, which exists)
       int inst[kShadeWin];
 #pragma unroll
-      for (uint32_t u = 0; u < kShadeWin; ++u) {
-        const uint32_t off = off0 + u * 64u;
-        inst[u] = __float_as_int(B.hit[hbase + (u < nu && off < cnt0 ? off : 0u)].w);
-      }
+      for (uint32_t u = 0; u < kShadeWin; ++u)
+        inst[u] = __float_as_int(B.hit[u < nu && lane < ucnt[u] ? upos[u] + lane : upos[0]].w);
       uint32_t key[kShadeWin];
 #pragma unroll
       for (uint32_t u = 0; u < kShadeWin; ++u) {
-        const uint32_t off = off0 + u * 64u;
         key[u] = kShadeInvalid;
-        if (u < nu && off < cnt0) key[u] = inst[u] < 0 ? static_cast<uint32_t>(kShadeMiss) : S0.ikind[inst[u]];
+        if (u < nu && lane < ucnt[u]) key[u] = inst[u] < 0 ? static_cast<uint32_t>(kShadeMiss) : S0.ikind[inst[u]];
       }
       uint32_t base = 0;
       for (uint32_t k = 0; k < kNumShadeClasses; ++k) {
@@ -291,11 +308,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
           base += static_cast<uint32_t>(__popcll(m));
         }
       }
-      // the window's positions are read by other lanes of this wave
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+    // the window's unit table and positions are read by other lanes of this
+    // wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     for (uint32_t p = 0; p < nu; ++p) {
     uint32_t pu = p, pl = lane;
     if (sorted) {
@@ -307,8 +325,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
     const DevChunk& ck = v.ch[ci];
     const DevJob& J = job_rec(B.jobs, uniform(ck.job));
     const DevScene& S = J.S;
-    const uint32_t off = (u0 + pu - v.prefix[ci]) * 64u + pl;
-    const uint32_t i = static_cast<uint32_t>(ck.base) + off;
+    const uint2 ui = uinfo[pu];
+    const uint32_t i = ui.x + pl;   // queue entry
     bool want_a = false, want_s = false, want_n = false;
     f3 ao{}, ad{}, ac{}, so{}, sd{}, sc{};
     float atmax = 0.f;
@@ -318,7 +336,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
     uint32_t slot = 0u;
     uint32_t nrec = kNoRecord;
     Rng rng{0};
-    if (off < v.cnt[ci]) {
+    if (pl < ui.y) {
       const float4 r0 = q.ray0[i], r1 = q.ray1[i], th = q.thr[i];
       const uint4 ps = q.st[i];
       const f3 o{r0.x, r0.y, r0.z}, d{r0.w, r1.x, r1.y};
@@ -371,7 +389,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
           // depth 0 a pixel's samples share its first sample's mixture (see
           // k_guide_infer); an entry without a valid mixture is not guided.
           const float alpha = kGuided ? *B.alpha_now : 0.f;
-          const uint32_t mi = (kGuided && depth == 0) ? static_cast<uint32_t>(ck.base) + off % static_cast<uint32_t>(ck.npix) : i;
+          // depth 0: segment 0 in slot order, so entry - qbase is the
+          // sample-major slot offset and the pixel's first sample is at
+          // qbase + pixel
+          const uint32_t mi = (kGuided && depth == 0)
+                                  ? static_cast<uint32_t>(ck.qbase) + (i - static_cast<uint32_t>(ck.qbase)) % static_cast<uint32_t>(ck.npix)
+                                  : i;
           const bool guided = kGuided && depth < B.guide_depth && lambert_exact(m) &&
                               B.mix[8 * B.mix_stride + mi].x >= 0.f;
           // area-light NEE (cpu_tiled.cpp:88-139)
@@ -526,13 +549,28 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
       }
     }
     // convergent section: compacted appends into this chunk's segments
-    const uint32_t sbase = 2u * static_cast<uint32_t>(ck.base);
-    uint32_t ia, is;
-    wave_append2(&B.counters[kShCnt + ci], want_a, want_s, ia, is);
-    ia += sbase;
-    is += sbase;
-    const uint32_t in = static_cast<uint32_t>(ck.base) +
-                        wave_append(&B.counters[kCnt + (1 - qi) * kMaxChunks + ci], want_n);
+    // the work-group's segment of the chunk's next path queue and shadow
+    // queue
+    const uint32_t xs = blockIdx.x % kQSeg;
+    uint32_t ia, is, in;
+    wave_append_pass(&B.counters[ctr_shadow(ci, static_cast<int>(xs))], want_a, want_s, ia, is,
+                     &B.counters[ctr_path(1 - qi, ci, static_cast<int>(xs))], want_n, in);
+    {
+      // a segment holds what its group can append (kSegSlack); an append
+      // past its end would be an internal error: counted, dropped
+      const uint32_t qseg = static_cast<uint32_t>(ck.qseg);
+      const bool drop = (want_a && ia >= 2u * qseg) || (want_s && is >= 2u * qseg) || (want_n && in >= qseg);
+      if (__ballot(drop) != 0ull) {
+        if (lane == 0) atomicAdd(&B.counters[kSegOverflow], 1u);
+        if (ia >= 2u * qseg) want_a = false;
+        if (is >= 2u * qseg) want_s = false;
+        if (in >= qseg) want_n = false;
+      }
+      const uint32_t segpos = static_cast<uint32_t>(ck.qbase) + xs * qseg;
+      ia += 2u * segpos;
+      is += 2u * segpos;
+      in += segpos;
+    }
     if (want_a) {
       B.sh0[ia] = make_float4(ao.x, ao.y, ao.z, ad.x);
       B.sh1[ia] = make_float4(ad.y, ad.z, atmax, __uint_as_float(slot * 2u));
@@ -562,8 +600,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
 template <bool kCount, bool kPrims>
 __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int lds_depth) {
   __shared__ BatchView v;
-  const uint32_t* counts = B.counters + kShCnt;
-  load_batch(v, B, counts, false);
+  const uint32_t* counts = B.counters + ctr_shadow(0, 0);
+  load_batch(v, B, counts, false, 2u);
   TravStack st = make_stack(B.spill, lds_depth);
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;

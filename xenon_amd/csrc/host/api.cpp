@@ -71,6 +71,7 @@ struct GuideState {
   float alpha_host = 0.f;
   bool train = true;
   uint32_t mix_stride = 0;        // queue entries of a guided batch (stride of mix and feat)
+  uint32_t slot_cap = 0;          // paths (slots) of a guided batch
   uint32_t rec_cap = 0, rec_every = 0;
 };
 
@@ -536,21 +537,24 @@ int alloc_waves(xnh_ctx* c) {
   free_all(c->wave_allocs);
   const int npix = c->cfg.width * c->cfg.height;
   const size_t cap = static_cast<size_t>(npix) * c->nsamples;
+  // queue entries: the slots plus the per-chunk segment slack (dev_types.h)
+  const size_t qcap = cap + static_cast<size_t>(kQueueSlack);
   WaveBuffers& B = c->B;
   B = WaveBuffers{};
   B.capacity = static_cast<int32_t>(cap);
+  B.qcap = static_cast<int32_t>(qcap);
   int rc = XNH_OK;
   for (int k = 0; k < 2 && rc == XNH_OK; ++k) {
-    rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray0, cap);
-    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray1, cap);
-    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].thr, cap);
-    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].st, cap);
+    rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray0, qcap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].ray1, qcap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].thr, qcap);
+    if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.q[k].st, qcap);
   }
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit, cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit_slot, cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh0, 2 * cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh1, 2 * cap);
-  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh_c, 2 * cap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit, qcap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.hit_slot, qcap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh0, 2 * qcap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh1, 2 * qcap);
+  if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.sh_c, 2 * qcap);
   if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.l_term, cap);
   if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.l_nee, 2 * cap);
   if (rc == XNH_OK) rc = dev_alloc(c, c->wave_allocs, &B.counters, static_cast<size_t>(kNumCtr));
@@ -775,16 +779,22 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
   if (cap == 0)
     return fail(c, XNH_E_ALLOC, "no wavefront queues: a re-tuning failed and the previous tuning could not be "
                                 "restored (a later xnh_set_tuning that succeeds makes the context usable again)");
-  if (c->guide.on) cap = c->guide.mix_stride;
+  // queue entries a batch may use (its chunks' segments)
+  size_t qlimit = static_cast<size_t>(c->B.qcap);
+  if (c->guide.on) {
+    cap = c->guide.slot_cap;
+    qlimit = c->guide.mix_stride;
+  }
   // batch schedule: greedy, one chunk per job per batch
   std::vector<DevChunk> chunks;
   std::vector<int32_t> counts;
-  size_t used = 0;
+  size_t used = 0, qused = 0;
   int in_batch = 0;
   auto new_batch = [&]() {
     counts.push_back(0);
     chunks.resize(counts.size() * kMaxChunks);
     used = 0;
+    qused = 0;
     in_batch = 0;
   };
   int depth = 0;
@@ -808,6 +818,10 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
       ck.npix = static_cast<int32_t>(npix);
       ck.base = static_cast<int32_t>(used);
       ck.cap = static_cast<int32_t>(npix * ns);
+      ck.qseg = static_cast<int32_t>(qseg_for(ck.cap));
+      ck.qbase = static_cast<int32_t>(qused);
+      qused += static_cast<size_t>(kQSeg) * static_cast<size_t>(ck.qseg);
+      if (qused > qlimit) return fail(c, XNH_E_INVALID, "queue segments exceed the queue capacity");
       counts.back() = ++in_batch;
       used += npix * ns;
       remaining -= ns;
@@ -879,8 +893,9 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
   uint32_t ovf = 0;
   HIP_OR_FAIL(c, hipMemcpyAsync(st, c->B.stats, sizeof st, hipMemcpyDeviceToHost, s));
   HIP_OR_FAIL(c, hipMemcpyAsync(&ovf, c->B.counters + kOverflow, sizeof ovf, hipMemcpyDeviceToHost, s));
-  uint32_t dog = 0;
+  uint32_t dog = 0, segovf = 0;
   HIP_OR_FAIL(c, hipMemcpyAsync(&dog, c->B.counters + kWatchdog, sizeof dog, hipMemcpyDeviceToHost, s));
+  HIP_OR_FAIL(c, hipMemcpyAsync(&segovf, c->B.counters + kSegOverflow, sizeof segovf, hipMemcpyDeviceToHost, s));
   HIP_OR_FAIL(c, hipStreamSynchronize(s));
   xnh_stats& o = c->stats;
   o = xnh_stats{};
@@ -914,6 +929,8 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
     }
   }
   if (dog) return fail(c, XNH_E_HIP, "traversal watchdog: " + std::to_string(dog) + " wave(s) reached the iteration bound");
+  if (segovf)
+    return fail(c, XNH_E_HIP, "queue segment overflow: " + std::to_string(segovf) + " append(s) dropped (internal error)");
   if (ovf) return fail(c, XNH_E_OVERFLOW, "traversal stack overflow");
   return XNH_OK;
 }
@@ -1593,8 +1610,10 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   // records: one per guided vertex of a recording path
   const size_t RC = std::min(static_cast<size_t>(cfg.record_paths) * static_cast<size_t>(cfg.guide_depth),
                              static_cast<size_t>(1) << 25);
-  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.mix, size_t(kMixVecs) * gcap);
-  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.feat, size_t(6) * gcap);
+  // per queue entry of a guided batch: its slots and the segment slack
+  const size_t gq = std::min(static_cast<size_t>(c->B.qcap), gcap + static_cast<size_t>(kQueueSlack));
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.mix, size_t(kMixVecs) * gq);
+  if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.feat, size_t(6) * gq);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.rec, 4 * RC);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.lam_all, RC);
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.lam_sum_all, size_t(1));
@@ -1612,7 +1631,8 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   g.net.grid = g.grid_h;
   g.net.w = g.w_h;
   g.net.wt = g.w_t;
-  g.mix_stride = static_cast<uint32_t>(gcap);
+  g.mix_stride = static_cast<uint32_t>(gq);
+  g.slot_cap = static_cast<uint32_t>(gcap);
   g.rec_cap = static_cast<uint32_t>(RC);
   g.rec_every = static_cast<uint32_t>(std::max<size_t>(1, gcap / static_cast<size_t>(cfg.record_paths)));
   g.alpha_host = cfg.warmup_steps > 0 ? 0.f : cfg.guide_fraction;
