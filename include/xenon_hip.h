@@ -265,7 +265,8 @@ typedef struct {
   uint32_t seed;           /* parameter initialisation */
   float bounds_lo[3], bounds_hi[3];   /* scene box mapped to the unit cube */
   int32_t train_batch;     /* training samples per batch (0: 2^18) */
-  int32_t record_paths;    /* paths per batch that record training samples (0: all) */
+  int32_t record_paths;    /* paths per batch that record training samples (0: all; at most
+                              2^25 / guide_depth, a hashed choice of the batch's paths) */
   int32_t train_steps;     /* optimisation steps per batch, each on an equal slice of its records
                               (0: one per 2^22 paths of a guided batch, 1 to 64; guided batches hold at
                               most 2^26 paths, or one job's pixels if more) */

@@ -186,7 +186,7 @@ struct WaveBuffers {
   uint32_t mix_stride;  // queue entries a guided batch uses at most (<= capacity): the stride of mix and of
                         // the hash-grid encodings
   float4* rec;      // [4][rec_cap] training records, see guide.h
-  uint32_t rec_cap;
+  uint32_t rec_cap;  // a multiple of kQSeg
   uint32_t rec_every;   // a path records its vertices when hash(slot, batch) % rec_every == 0
   float guide_alpha;    // largest one-sample MIS probability of the guide; 0 = guiding off
   const float* alpha_now;   // the probability used by the current batch (device memory)
@@ -196,19 +196,27 @@ struct WaveBuffers {
 enum Ctr : int {
   kBatch = 0,                            // batch being rendered (advanced on the device)
   kOverflow = 1,                         // lanes whose stack exceeded kMaxStack
-  kRecCnt = 2,                           // guiding training records of the batch
   kWatchdog = 3,                         // waves that hit an iteration bound (see dtrace.h)
   kSegOverflow = 4,                      // appends past a queue segment's end (dropped; never expected)
   kCnt = 16,                                        // [queue][chunk][segment] live paths (stride kCtrStride)
   kShCnt = kCnt + 2 * kMaxChunks * kQSeg * kCtrStride,  // [chunk][segment] shadow rays
   kCursor = kShCnt + kMaxChunks * kQSeg * kCtrStride,   // per-launch work cursors of a batch
   kNumCursors = 2 * 64 + 4,
-  kNumCtr = kCursor + kNumCursors
+  // [segment] guiding training records of the batch (stride kCtrStride)
+  kRecCnt = (kCursor + kNumCursors + kCtrStride - 1) / kCtrStride * kCtrStride,
+  kNumCtr = kRecCnt + kQSeg * kCtrStride
 };
 
 // Counter of segment x of chunk ci: path queue q, shadow queue.
 constexpr int ctr_path(int q, int ci, int x) { return kCnt + ((q * kMaxChunks + ci) * kQSeg + x) * kCtrStride; }
 constexpr int ctr_shadow(int ci, int x) { return kShCnt + (ci * kQSeg + x) * kCtrStride; }
+// 64-entry units per window of the shading kernel
+constexpr uint32_t kShadeWin = 8;
+
+// Counter of training-record segment x; the records are kQSeg segments of
+// rec_cap / kQSeg, segment x from x * (rec_cap / kQSeg). A shading window
+// appends its records to segment (window index) % kQSeg.
+constexpr int ctr_rec(int x) { return kRecCnt + x * kCtrStride; }
 
 enum Stat : int {
   kStPaths = 0,

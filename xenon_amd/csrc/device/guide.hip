@@ -336,17 +336,26 @@ __device__ __forceinline__ bool record_drawn(uint32_t r, float l, float tau, uin
   return static_cast<float>(h >> 8) * (1.f / 16777216.f) < p;
 }
 
-// After a guided batch, pass 1 over its training records (grid-stride):
-// the lambda of every record into lam_all, their sum into *lam_sum_all (one
-// atomic per work-group).
+// The records of segment x in use: the first min(count, capacity) of it.
+__device__ __forceinline__ uint32_t rec_used(const WaveBuffers& B, int x) {
+  return min(B.counters[ctr_rec(x)], B.rec_cap / kQSeg);
+}
+
+// After a guided batch, pass 1 over its training records (grid-stride over
+// each segment's records in use): the lambda of every record into lam_all,
+// their sum into *lam_sum_all (one atomic per work-group).
 __global__ void __launch_bounds__(256) k_guide_lambda(WaveBuffers B, float* lam_all, float* lam_sum_all) {
   __shared__ float red[4];
-  const uint32_t n = min(B.counters[kRecCnt], B.rec_cap);
+  const uint32_t rs = B.rec_cap / kQSeg;
   float acc = 0.f;
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
-    const float l = record_lambda(B, r);
-    lam_all[r] = l;
-    acc += l;
+  for (int x = 0; x < kQSeg; ++x) {
+    const uint32_t n = rec_used(B, x);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+      const uint32_t r = static_cast<uint32_t>(x) * rs + j;
+      const float l = record_lambda(B, r);
+      lam_all[r] = l;
+      acc += l;
+    }
   }
   const float t = block_sumf(acc, red);
   if (threadIdx.x == 0 && t > 0.f) atomicAdd(lam_sum_all, t);
@@ -367,13 +376,17 @@ __global__ void __launch_bounds__(256) k_guide_select(WaveBuffers B, const float
   __shared__ uint32_t red[5];
   __shared__ float redf[4];
   __shared__ uint32_t base_sh;
-  const uint32_t n = min(B.counters[kRecCnt], B.rec_cap);
+  const uint32_t rs = B.rec_cap / kQSeg;
   const float tau = *lam_sum_all / target;
   const uint32_t r0 = blockIdx.x * blockDim.x + threadIdx.x, stride = gridDim.x * blockDim.x;
   uint32_t mine = 0;
-  for (uint32_t r = r0; r < n; r += stride) {
-    float w;
-    if (record_drawn(r, lam_all[r], tau, seed, w)) ++mine;
+  for (int x = 0; x < kQSeg; ++x) {
+    const uint32_t n = rec_used(B, x);
+    for (uint32_t j = r0; j < n; j += stride) {
+      const uint32_t r = static_cast<uint32_t>(x) * rs + j;
+      float w;
+      if (record_drawn(r, lam_all[r], tau, seed, w)) ++mine;
+    }
   }
   uint32_t total;
   const uint32_t before = block_excl_scan(mine, red, &total);
@@ -382,22 +395,26 @@ __global__ void __launch_bounds__(256) k_guide_select(WaveBuffers B, const float
   uint32_t k = base_sh + before;
   float added = 0.f;
   const uint32_t R = B.rec_cap;
-  for (uint32_t r = r0; r < n && k < cap; r += stride) {
-    float w;
-    if (!record_drawn(r, lam_all[r], tau, seed, w)) continue;
-    const float4 a = B.rec[r], b = B.rec[R + r], c = B.rec[2 * R + r];
-    pos[3 * k] = a.x;
-    pos[3 * k + 1] = a.y;
-    pos[3 * k + 2] = a.z;
-    nrm[3 * k] = b.x;
-    nrm[3 * k + 1] = b.y;
-    nrm[3 * k + 2] = b.z;
-    dir[3 * k] = c.x;
-    dir[3 * k + 1] = c.y;
-    dir[3 * k + 2] = c.z;
-    lam[k] = w;
-    added += w;
-    ++k;
+  for (int x = 0; x < kQSeg && k < cap; ++x) {
+    const uint32_t n = rec_used(B, x);
+    for (uint32_t j = r0; j < n && k < cap; j += stride) {
+      const uint32_t r = static_cast<uint32_t>(x) * rs + j;
+      float w;
+      if (!record_drawn(r, lam_all[r], tau, seed, w)) continue;
+      const float4 a = B.rec[r], b = B.rec[R + r], c = B.rec[2 * R + r];
+      pos[3 * k] = a.x;
+      pos[3 * k + 1] = a.y;
+      pos[3 * k + 2] = a.z;
+      nrm[3 * k] = b.x;
+      nrm[3 * k + 1] = b.y;
+      nrm[3 * k + 2] = b.z;
+      dir[3 * k] = c.x;
+      dir[3 * k + 1] = c.y;
+      dir[3 * k + 2] = c.z;
+      lam[k] = w;
+      added += w;
+      ++k;
+    }
   }
   const float t = block_sumf(added, redf);
   if (threadIdx.x == 0 && t > 0.f) atomicAdd(lam_sum, t);

@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
     B.counters[ctr_shadow(ci, x)] = 0u;
   }
   if (tid < kNumCursors) B.counters[kCursor + tid] = 0u;
-  if (tid == 0) B.counters[kRecCnt] = 0u;
+  if (tid < kQSeg) B.counters[ctr_rec(tid)] = 0u;
   if (v.nch == 0) return;
   const int end = v.ch[v.nch - 1].base + v.ch[v.nch - 1].cap;
   const int stride = static_cast<int>(gridDim.x * blockDim.x);
@@ -240,12 +240,11 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
 // neither its result nor where it is written (per-slot radiance terms;
 // appended rays carry their slot).
 // ---------------------------------------------------------------------------
-// Units per shading window. Eight sort better than four and amortise the
-// window's class loads over more entries; at four waves per SIMD (128 VGPRs,
-// a few bytes spilled) the plain kernel beat three waves without the spill
-// (benchmark k_shade 225 -> 217 ms per two steps). The guided kernel keeps
-// its three waves.
-constexpr uint32_t kShadeWin = 8;
+// Units per shading window (kShadeWin). Eight sort better than four and
+// amortise the window's class loads over more entries; at four waves per
+// SIMD (128 VGPRs, a few bytes spilled) the plain kernel beat three waves
+// without the spill (benchmark k_shade 225 -> 217 ms per two steps). The
+// guided kernel keeps its three waves.
 
 template <bool kGuided>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuided ? 3 : 4))) k_shade(WaveBuffers B, int qi, int depth) {
@@ -501,9 +500,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
                 alive = false;
               } else {
                 const bool rec = B.rec_every > 0u && record_path(slot, v.batch, B.rec_every);
-                const uint32_t r = wave_append(&B.counters[kRecCnt], rec);   // one atomic per wave
+                // one atomic per wave, on the record segment of this window
+                // (windows go to the segments in turn, so no segment takes
+                // more than one window's entries over an equal share)
+                const uint32_t rs = B.rec_cap / kQSeg, rx = (u0 / kShadeWin) % kQSeg;
+                uint32_t r = wave_append(&B.counters[ctr_rec(static_cast<int>(rx))], rec);
                 if (rec) {
-                  if (r < B.rec_cap) {
+                  if (r < rs) {
+                    r += rx * rs;
                     const uint32_t R = B.rec_cap;
                     const float nan = __builtin_nanf("");
                     B.rec[r] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, T.x);
@@ -816,8 +820,12 @@ hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int curso
   return hipGetLastError();
 }
 
+// Past the guided depths the guided kernel takes the plain kernel's paths
+// exactly: no entry is guided there, and the training records a guided vertex
+// starts are completed by the next depth's shading. Those depths run the
+// plain kernel, which keeps more waves per SIMD.
 hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, int grid) {
-  if (B.guide_alpha > 0.f) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, B, qi, depth);
+  if (B.guide_alpha > 0.f && depth <= B.guide_depth) hipLaunchKernelGGL(k_shade<true>, dim3(grid), dim3(256), 0, s, B, qi, depth);
   else hipLaunchKernelGGL(k_shade<false>, dim3(grid), dim3(256), 0, s, B, qi, depth);
   return hipGetLastError();
 }

@@ -1607,9 +1607,18 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.w_t, size_t(kWTotal));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.loss, size_t(1));
   const size_t R = static_cast<size_t>(cfg.train_batch);
-  // records: one per guided vertex of a recording path
-  const size_t RC = std::min(static_cast<size_t>(cfg.record_paths) * static_cast<size_t>(cfg.guide_depth),
-                             static_cast<size_t>(1) << 25);
+  // records: one per guided vertex of a recording path, at most 2^25 a
+  // batch. The recording paths are a hashed 1-in-rec_every choice of the
+  // batch's paths, rec_every the smallest that keeps their expected number
+  // within that (so the records cover the whole image rather than its
+  // first-shaded part). kQSeg equal segments, each with room over an equal
+  // share for one shading window's entries per guided depth and 1/64 for
+  // the draw's spread, so no record is dropped in practice.
+  const size_t rp = std::min(static_cast<size_t>(cfg.record_paths),
+                             (static_cast<size_t>(1) << 25) / static_cast<size_t>(cfg.guide_depth));
+  size_t RC = rp * static_cast<size_t>(cfg.guide_depth);
+  RC += RC / 64 + static_cast<size_t>(kQSeg) * kShadeWin * 64 * static_cast<size_t>(cfg.guide_depth);
+  RC = (RC + kQSeg - 1) / kQSeg * kQSeg;
   // per queue entry of a guided batch: its slots and the segment slack
   const size_t gq = std::min(static_cast<size_t>(c->B.qcap), gcap + static_cast<size_t>(kQueueSlack));
   if (rc == XNH_OK) rc = dev_alloc(c, g.allocs, &g.mix, size_t(kMixVecs) * gq);
@@ -1634,7 +1643,7 @@ int xnh_guide_enable(xnh_ctx* c, const xnh_guide_config* cfg_in) {
   g.mix_stride = static_cast<uint32_t>(gq);
   g.slot_cap = static_cast<uint32_t>(gcap);
   g.rec_cap = static_cast<uint32_t>(RC);
-  g.rec_every = static_cast<uint32_t>(std::max<size_t>(1, gcap / static_cast<size_t>(cfg.record_paths)));
+  g.rec_every = static_cast<uint32_t>(std::max<size_t>(1, (gcap + rp - 1) / rp));
   g.alpha_host = cfg.warmup_steps > 0 ? 0.f : cfg.guide_fraction;
   if (hipMemcpy(g.alpha_dev, &g.alpha_host, sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
     xnh_guide_disable(c);
