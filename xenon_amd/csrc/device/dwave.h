@@ -55,6 +55,40 @@ __device__ __forceinline__ int tiled_pixel(int t, int W, int H) {
   return (8 * band + yi) * W + 8 * bx + (q - yi * wb);
 }
 
+// The camera path behind depth-0 queue entry e of chunk ck: depth 0 holds a
+// chunk's paths as its segment 0 in slot order, sample-major over the tiled
+// pixels (slot = base + e - qbase). Its seed is the pixel index plus the
+// sample index times 0xdeadbeef, its ray the jittered pinhole ray
+// (cpu_tiled.cpp:20-29, camera/camera.h:35-37), and rng is the state after
+// the two jitter draws. Camera rays are not stored: the depth-0 traversal,
+// shading and guide inference each rebuild them here from the entry index,
+// bit for bit the same wherever they are made.
+struct CameraPath {
+  f3 o, d;
+  Rng rng;
+  uint32_t slot;
+};
+__device__ __forceinline__ CameraPath camera_path(const DevChunk& ck, const DevJob& J, uint32_t e) {
+  const int local = static_cast<int>(e) - ck.qbase;
+  const int s = local / ck.npix;
+  const int pixel = tiled_pixel(local - s * ck.npix, J.W, J.H);
+  const int spp = ck.spp_first + s;
+  const uint64_t seed = static_cast<uint64_t>(pixel) + static_cast<uint64_t>(spp) * 0xdeadbeefULL;
+  CameraPath p;
+  p.rng = rng_for(seed);
+  const int x = pixel % J.W, y = pixel / J.W;
+  const float fx = static_cast<float>(x) + p.rng.f();
+  const float fy = static_cast<float>(y) + p.rng.f();
+  const float u = fx / static_cast<float>(J.W), w = fy / static_cast<float>(J.H);
+  p.o = f3{J.cam.eye[0], J.cam.eye[1], J.cam.eye[2]};
+  const f3 llc{J.cam.llc[0], J.cam.llc[1], J.cam.llc[2]};
+  const f3 hz{J.cam.horiz[0], J.cam.horiz[1], J.cam.horiz[2]};
+  const f3 vt{J.cam.vert[0], J.cam.vert[1], J.cam.vert[2]};
+  p.d = normalize(llc + u * hz + w * vt - p.o);
+  p.slot = static_cast<uint32_t>(ck.base + local);
+  return p;
+}
+
 // Wave-aggregated append: one atomic per wave, ranks by ballot prefix.
 // Must be reached by every lane of the wave.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool want) {

@@ -260,9 +260,18 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_encode(WaveBuffers B
     const float4 hv = B.hit[u.i];
     const int inst = __float_as_int(hv.w);
     if (inst >= 0) {
-      const float4 r0 = q.ray0[u.i], r1 = q.ray1[u.i];
+      f3 o, d;
+      if (depth == 0) {   // camera rays are not stored
+        const CameraPath cp = camera_path(*u.ck, job_rec(B.jobs, uniform(u.ck->job)), u.i);
+        o = cp.o;
+        d = cp.d;
+      } else {
+        const float4 r0 = q.ray0[u.i], r1 = q.ray1[u.i];
+        o = f3{r0.x, r0.y, r0.z};
+        d = f3{r0.w, r1.x, r1.y};
+      }
       const HitRef h{hv.x, hv.y, hv.z, B.hit_slot[u.i], inst};
-      const GeoSurface sf = resolve_geo(S, h, f3{r0.x, r0.y, r0.z}, f3{r0.w, r1.x, r1.y}, kBounceOffset);
+      const GeoSurface sf = resolve_geo(S, h, o, d, kBounceOffset);
       const bool emitter = sf.lmap >= 0 && S.lmap[sf.lmap + static_cast<int>(sf.tri)] >= 0;
       if (!emitter && lambert_exact(load_mat(S.mats, sf.mat))) {
         want = true;

@@ -28,10 +28,12 @@
 namespace xnh {
 
 // ---------------------------------------------------------------------------
-// raygen: per-(pixel, sample) seed and jittered pinhole ray
-// (cpu_tiled.cpp:20-29, camera/camera.h:35-37); resets the batch's queue
-// counters and work cursors. Grid-stride over the batch's paths, so the
-// grid is bounded whatever the queue capacity.
+// raygen: starts a batch. Depth 0 of every chunk is its camera paths in slot
+// order (segment 0 counts the chunk's paths; the rays themselves are rebuilt
+// from the entry index by camera_path where they are read); the batch's
+// other queue and shadow counters, work cursors and record counters are
+// reset and every slot's radiance terms cleared. Grid-stride over the
+// batch's slots, so the grid is bounded whatever the queue capacity.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
   __shared__ BatchView v;
@@ -54,39 +56,10 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
   if (tid < kNumCursors) B.counters[kCursor + tid] = 0u;
   if (tid < kQSeg) B.counters[ctr_rec(tid)] = 0u;
   if (v.nch == 0) return;
+  // the chunks' slots are consecutive from 0
   const int end = v.ch[v.nch - 1].base + v.ch[v.nch - 1].cap;
   const int stride = static_cast<int>(gridDim.x * blockDim.x);
-  const PathQueue& q = B.q[0];
   for (int gid = tid; gid < end; gid += stride) {
-    int lo = 0, hi = v.nch;
-    while (hi - lo > 1) {
-      const int mid = (lo + hi) >> 1;
-      if (v.ch[mid].base <= gid) lo = mid;
-      else hi = mid;
-    }
-    const DevChunk& ck = v.ch[lo];
-    const DevJob& J = B.jobs[ck.job];
-    const int local = gid - ck.base;
-    const int s = local / ck.npix;
-    const int pixel = tiled_pixel(local - s * ck.npix, J.W, J.H);
-    const int spp = ck.spp_first + s;
-    const uint64_t seed = static_cast<uint64_t>(pixel) + static_cast<uint64_t>(spp) * 0xdeadbeefULL;
-    Rng rng = rng_for(seed);
-    const int x = pixel % J.W, y = pixel / J.W;
-    const float fx = static_cast<float>(x) + rng.f();
-    const float fy = static_cast<float>(y) + rng.f();
-    const float u = fx / static_cast<float>(J.W), w = fy / static_cast<float>(J.H);
-    const f3 eye{J.cam.eye[0], J.cam.eye[1], J.cam.eye[2]};
-    const f3 llc{J.cam.llc[0], J.cam.llc[1], J.cam.llc[2]};
-    const f3 hz{J.cam.horiz[0], J.cam.horiz[1], J.cam.horiz[2]};
-    const f3 vt{J.cam.vert[0], J.cam.vert[1], J.cam.vert[2]};
-    const f3 d = normalize(llc + u * hz + w * vt - eye);
-    const int e = ck.qbase + local;   // queue entry (segment 0)
-    q.ray0[e] = make_float4(eye.x, eye.y, eye.z, d.x);
-    q.ray1[e] = make_float4(d.y, d.z, 0.f, __uint_as_float(1u));
-    q.thr[e] = make_float4(1.f, 1.f, 1.f, 0.f);
-    q.st[e] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), static_cast<uint32_t>(gid),
-                         kNoRecord);
     B.l_term[gid] = make_float4(0.f, 0.f, 0.f, 0.f);
     B.l_nee[2 * gid] = make_float4(0.f, 0.f, 0.f, 0.f);
     B.l_nee[2 * gid + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -170,7 +143,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
           my = pbase + pnext + rank;
           f3 o, d;
           float tmin, tmax;
-          load(my, o, d, tmin, tmax);
+          load(my, pci, o, d, tmin, tmax);
           w.begin(S, o, d, tmin, tmax, st);
         }
         pnext = min(pend, pnext + static_cast<uint32_t>(__popcll(idle)));
@@ -192,9 +165,10 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
 
 // ---------------------------------------------------------------------------
 // closest: persistent closest-hit traversal of queue qi (kCount: with the
-// traversal work counters).
+// traversal work counters; kCamera: depth 0, whose rays are the chunks'
+// camera rays, rebuilt from the entry index).
 // ---------------------------------------------------------------------------
-template <bool kCount, bool kPrims>
+template <bool kCount, bool kPrims, bool kCamera>
 __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int cursor, int lds_depth) {
   __shared__ BatchView v;
   const uint32_t* counts = B.counters + ctr_path(qi, 0, 0);
@@ -211,10 +185,17 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
   bool overflow = false;
   trace_queue<false, kCount, kPrims>(
       B, v, &B.counters[kCursor + cursor], 1u, st, c, rays, overflow,
-      [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
-        const float4 r0 = q.ray0[i], r1 = q.ray1[i];
-        o = f3{r0.x, r0.y, r0.z};
-        d = f3{r0.w, r1.x, r1.y};
+      [&](uint32_t i, int ci, f3& o, f3& d, float& tmin, float& tmax) {
+        if constexpr (kCamera) {
+          const DevChunk& ck = v.ch[ci];
+          const CameraPath p = camera_path(ck, job_rec(B.jobs, uniform(ck.job)), i);
+          o = p.o;
+          d = p.d;
+        } else {
+          const float4 r0 = q.ray0[i], r1 = q.ray1[i];
+          o = f3{r0.x, r0.y, r0.z};
+          d = f3{r0.w, r1.x, r1.y};
+        }
         tmin = kBounceOffset;
         tmax = __builtin_huge_valf();
       },
@@ -336,19 +317,38 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
     uint32_t nrec = kNoRecord;
     Rng rng{0};
     if (pl < ui.y) {
-      const float4 r0 = q.ray0[i], r1 = q.ray1[i], th = q.thr[i];
-      const uint4 ps = q.st[i];
-      const f3 o{r0.x, r0.y, r0.z}, d{r0.w, r1.x, r1.y};
-      const float prev_pdf = r1.z;
-      const bool prev_delta = (__float_as_uint(r1.w) & 1u) != 0u;
-      f3 T{th.x, th.y, th.z};
-      rng.s = static_cast<uint64_t>(ps.x) | (static_cast<uint64_t>(ps.y) << 32);
-      slot = ps.z;
-      if (kGuided && ps.w < B.rec_cap) {
+      f3 o, d, T;
+      float prev_pdf;
+      bool prev_delta;
+      uint32_t prec = kNoRecord;   // the training record the previous vertex started
+      if (depth == 0) {
+        // a camera path: unit throughput, no previous vertex (the emitter
+        // weight below is 1 at depth 0)
+        const CameraPath cp = camera_path(ck, J, i);
+        o = cp.o;
+        d = cp.d;
+        T = f3{1.f, 1.f, 1.f};
+        prev_pdf = 0.f;
+        prev_delta = true;
+        rng = cp.rng;
+        slot = cp.slot;
+      } else {
+        const float4 r0 = q.ray0[i], r1 = q.ray1[i], th = q.thr[i];
+        const uint4 ps = q.st[i];
+        o = f3{r0.x, r0.y, r0.z};
+        d = f3{r0.w, r1.x, r1.y};
+        prev_pdf = r1.z;
+        prev_delta = (__float_as_uint(r1.w) & 1u) != 0u;
+        T = f3{th.x, th.y, th.z};
+        rng.s = static_cast<uint64_t>(ps.x) | (static_cast<uint64_t>(ps.y) << 32);
+        slot = ps.z;
+        prec = ps.w;
+      }
+      if (kGuided && prec < B.rec_cap) {
         // the previous vertex recorded a training sample: note the path's
         // radiance so far (its NEE included) so that what follows is known
         const float4 a = B.l_nee[2 * slot], b = B.l_nee[2 * slot + 1];
-        B.rec[3 * B.rec_cap + ps.w] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, __uint_as_float(slot));
+        B.rec[3 * B.rec_cap + prec] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, __uint_as_float(slot));
       }
       const float4 hv = B.hit[i];
       HitRef h{hv.x, hv.y, hv.z, B.hit_slot[i], __float_as_int(hv.w)};
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
   bool overflow = false;
   trace_queue<true, kCount, kPrims>(
       B, v, &B.counters[kCursor + cursor], 2u, st, c, rays, overflow,
-      [&](uint32_t i, f3& o, f3& d, float& tmin, float& tmax) {
+      [&](uint32_t i, int, f3& o, f3& d, float& tmin, float& tmax) {
         const float4 r0 = B.sh0[i], r1 = B.sh1[i];
         o = f3{r0.x, r0.y, r0.z};
         d = f3{r0.w, r1.x, r1.y};
@@ -783,7 +783,7 @@ int trace_grid(int block, int lds_depth) {
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
   int per_cu = 0;
   const size_t lds = static_cast<size_t>(block) * lds_depth * sizeof(uint32_t);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_closest<false, false>), block, lds) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_closest<false, false, false>), block, lds) !=
       hipSuccess)
     return -1;
   if (per_cu < 1) per_cu = 1;
@@ -807,15 +807,24 @@ hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths) {
   return hipGetLastError();
 }
 
+template <bool kCount, bool kPrims>
+void launch_closest_kind(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block, int lds_depth,
+                         size_t lds, bool camera) {
+  if (camera)
+    hipLaunchKernelGGL((k_closest<kCount, kPrims, true>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+  else
+    hipLaunchKernelGGL((k_closest<kCount, kPrims, false>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+}
+
 hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block,
-                          int lds_depth, bool count, bool prims) {
+                          int lds_depth, bool count, bool prims, bool camera) {
   const size_t lds = static_cast<size_t>(block) * lds_depth * 4;
   if (prims) {
-    if (count) hipLaunchKernelGGL((k_closest<true, true>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
-    else hipLaunchKernelGGL((k_closest<false, true>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+    if (count) launch_closest_kind<true, true>(s, B, qi, cursor, grid, block, lds_depth, lds, camera);
+    else launch_closest_kind<false, true>(s, B, qi, cursor, grid, block, lds_depth, lds, camera);
   } else {
-    if (count) hipLaunchKernelGGL((k_closest<true, false>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
-    else hipLaunchKernelGGL((k_closest<false, false>), dim3(grid), dim3(block), lds, s, B, qi, cursor, lds_depth);
+    if (count) launch_closest_kind<true, false>(s, B, qi, cursor, grid, block, lds_depth, lds, camera);
+    else launch_closest_kind<false, false>(s, B, qi, cursor, grid, block, lds_depth, lds, camera);
   }
   return hipGetLastError();
 }

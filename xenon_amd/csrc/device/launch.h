@@ -13,8 +13,9 @@ int trace_grid(int block, int lds_depth);
 int num_cus();
 
 hipError_t launch_raygen(hipStream_t s, const WaveBuffers& B, int max_paths);
+// camera: depth 0 (the queue's rays are the chunks' camera rays)
 hipError_t launch_closest(hipStream_t s, const WaveBuffers& B, int qi, int cursor, int grid, int block,
-                          int lds_depth, bool count, bool prims);
+                          int lds_depth, bool count, bool prims, bool camera);
 hipError_t launch_shade(hipStream_t s, const WaveBuffers& B, int qi, int depth, int grid);
 hipError_t launch_shadow(hipStream_t s, const WaveBuffers& B, int cursor, int grid, int block, int lds_depth,
                          bool count, bool prims);

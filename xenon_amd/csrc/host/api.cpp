@@ -631,7 +631,8 @@ int enqueue_batch(xnh_ctx* c, int depth, size_t* ev) {
   for (int d = 0; d < depth; ++d) {
     const int qi = d & 1;
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
-    HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth, c->counting, c->prims));
+    HIP_OR_FAIL(c, launch_closest(s, c->B, qi, 2 * d, c->grid_trace, c->block, c->lds_depth, c->counting, c->prims,
+                                  d == 0));
     if (ev) HIP_OR_FAIL(c, hipEventRecord(c->prof_events[(*ev)++], s));
     if (c->guide.on && d < c->B.guide_depth)
       HIP_OR_FAIL(c, launch_guide_infer(s, c->B, qi, d, c->guide.net, c->guide.feat, c->grid_shade * 2));
