@@ -13,8 +13,9 @@ K timed steps bracketed by device synchronisation. `value` = pixels x samples
 / seconds / 1e6 over the timed steps.
 
 roofline: the closest-hit traversal kernel. Its algorithmic bytes per ray are
-32 B ray in + 16 B hit out, plus 32 B per BVH node box tested, 36 B per
-triangle test and 96 B per instance visit; achieved = those bytes / the
+32 B ray in (bounce rays only: camera rays are generated in the kernel from
+their queue index) + 16 B hit out, plus 32 B per BVH node box tested, 36 B
+per triangle test and 96 B per instance visit; achieved = those bytes / the
 kernel's summed launch time measured with HIP events on the library's stream
 during the timed steps. The timed steps run the production kernels, which
 keep no work counters; the node / triangle / instance counts come from
@@ -55,9 +56,9 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(st: dict, kind: str = "closest") -> float:
+def algorithmic_bytes(st: dict, kind: str = "closest", camera_rays: int = 0) -> float:
     if kind == "closest":
-        return (48.0 * st["closest_rays"] + 32.0 * st["node_tests"] + 36.0 * st["tri_tests"]
+        return (48.0 * st["closest_rays"] - 32.0 * camera_rays + 32.0 * st["node_tests"] + 36.0 * st["tri_tests"]
                 + 96.0 * st["inst_visits"])
     return (36.0 * st["shadow_rays"] + 32.0 * st["shadow_node_tests"] + 36.0 * st["shadow_tri_tests"]
             + 96.0 * st["shadow_inst_visits"])
@@ -477,7 +478,8 @@ def main():
     paths = wl.paths * args.steps
     mpaths = paths / elapsed / 1e6
     mrays = (tot["closest_rays"] + tot["shadow_rays"]) / elapsed / 1e6
-    cbytes = algorithmic_bytes(tot, "closest")
+    camera_rays = wl.paths * args.steps   # one per path
+    cbytes = algorithmic_bytes(tot, "closest", camera_rays)
     launches = max(1, tot["closest_launches"])
     achieved = cbytes / (tot["closest_ms"] / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -486,12 +488,13 @@ def main():
                 "kernel": "k_closest (closest-hit traversal)",
                 "bytes_per_launch": round(cbytes / launches), "avg_launch_ms": round(tot["closest_ms"] / launches, 4)}
     sbytes = algorithmic_bytes(tot, "shadow")
-    # shading: path state (64 B) and hit (20 B) in, material record (64 B),
-    # radiance term (16 B) out per shaded entry; the continuation (64 B) of
-    # every closest-hit ray after the camera rays; one shadow record (48 B)
-    # per shadow ray
-    hbytes = (164.0 * tot["closest_rays"] + 64.0 * max(0, tot["closest_rays"] - wl.paths * args.steps)
-              + 48.0 * tot["shadow_rays"])
+    # shading: path state (64 B; not for camera paths, which are rebuilt
+    # from their index) and hit (20 B) in, material record (64 B), radiance
+    # term (16 B) out per shaded entry; the continuation (64 B) of every
+    # closest-hit ray after the camera rays; one shadow record (48 B) per
+    # shadow ray
+    bounce = max(0, tot["closest_rays"] - camera_rays)
+    hbytes = 100.0 * tot["closest_rays"] + (64.0 + 64.0) * bounce + 48.0 * tot["shadow_rays"]
     out = {
         "metric": METRIC,
         "value": round(mpaths, 3),
