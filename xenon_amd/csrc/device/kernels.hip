@@ -74,12 +74,19 @@ __global__ void __launch_bounds__(256) k_raygen(WaveBuffers B) {
 // ray. A unit lies in one chunk; the rays in flight in a wave always share
 // one job (and scene): a unit of another job waits until the wave drains.
 // seg: queue entries per chunk sample slot (1 path queue, 2 shadow queue).
+// kSort: the pool is all fetched units of one chunk (up to 16, 1024 rays)
+// at once, ordered by the direction octant of their rays (bits 1-3 of
+// ray1.w, set by the shading kernel), so that the rays a wave walks together
+// visit the BVH's children in the same order; slist and sbase are the wave's
+// LDS for the order and the units' first entries. Which rays share a wave
+// changes no ray's result.
 // Must be reached by every lane of the wave.
 // ---------------------------------------------------------------------------
-template <bool kAny, bool kCount, bool kPrims, class Load, class Finish>
+template <bool kAny, bool kCount, bool kPrims, bool kSort, class Load, class Finish>
 __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchView& v, uint32_t* cursor,
                                             uint32_t seg, TravStack& st, TravCount& c, uint32_t& rays,
-                                            bool& overflow, Load load, Finish finish) {
+                                            bool& overflow, const float4* ray1, uint16_t* slist, uint32_t* sbase,
+                                            Load load, Finish finish) {
   constexpr uint32_t kIdle = 0xFFFFFFFFu;
   const uint32_t total = v.prefix[v.nch];
   const int lane = lane_id();
@@ -127,11 +134,47 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
           const int ci = uniform(find_chunk(v, ucur));
           if (act == 0ull || v.ch[ci].job == v.ch[pci].job) {
             pci = ci;
-            uint32_t first, lim;
-            pbase = unit_segment(v, ci, ucur - v.prefix[ci], seg, first, lim);
-            pnext = first;
-            pend = min(first + 64u, lim);
-            ++ucur;
+            if constexpr (kSort) {
+              // the fetched units that lie in this chunk (1 to 16), keyed by
+              // octant (4 bits per unit and lane; 8 = no entry)
+              const uint32_t n = min(uend, v.prefix[ci + 1]) - ucur;
+              uint32_t keys0 = 0u, keys1 = 0u;
+              __builtin_amdgcn_wave_barrier();   // every lane has taken its entry of the last pool
+              for (uint32_t j = 0; j < n; ++j) {
+                uint32_t first, lim;
+                const uint32_t b = unit_segment(v, ci, ucur + j - v.prefix[ci], seg, first, lim) + first;
+                const uint32_t cnt = lim > first ? min(64u, lim - first) : 0u;
+                if (lane == 0) sbase[j] = b;
+                uint32_t key = 8u;
+                if (static_cast<uint32_t>(lane) < cnt)
+                  key = (reinterpret_cast<const uint32_t*>(ray1)[4u * (b + static_cast<uint32_t>(lane)) + 3u] >> 1) & 7u;
+                if (j < 8u) keys0 |= key << (4u * j);
+                else keys1 |= key << (4u * (j - 8u));
+              }
+              // counting sort: octant, then unit, then lane
+              uint32_t pos = 0;
+              for (uint32_t k = 0; k < 8u; ++k) {
+                for (uint32_t j = 0; j < n; ++j) {
+                  const uint32_t key = (j < 8u ? keys0 >> (4u * j) : keys1 >> (4u * (j - 8u))) & 15u;
+                  const unsigned long long m = __ballot(key == k);
+                  if (key == k) slist[pos + static_cast<uint32_t>(__popcll(m & below))] = static_cast<uint16_t>((j << 6) | static_cast<uint32_t>(lane));
+                  pos += static_cast<uint32_t>(__popcll(m));
+                }
+              }
+              // the order and the unit bases are read by other lanes
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+              pnext = 0;
+              pend = pos;
+              ucur += n;
+            } else {
+              uint32_t first, lim;
+              pbase = unit_segment(v, ci, ucur - v.prefix[ci], seg, first, lim);
+              pnext = first;
+              pend = min(first + 64u, lim);
+              ++ucur;
+            }
           }
         }
       }
@@ -140,7 +183,12 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
         const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
         const DevScene& S = job_scene(B.jobs, uniform(v.ch[pci].job));
         if (my == kIdle && pnext + rank < pend) {
-          my = pbase + pnext + rank;
+          if constexpr (kSort) {
+            const uint32_t t = slist[pnext + rank];
+            my = sbase[t >> 6] + (t & 63u);
+          } else {
+            my = pbase + pnext + rank;
+          }
           f3 o, d;
           float tmin, tmax;
           load(my, pci, o, d, tmin, tmax);
@@ -168,9 +216,22 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
 // traversal work counters; kCamera: depth 0, whose rays are the chunks'
 // camera rays, rebuilt from the entry index).
 // ---------------------------------------------------------------------------
+// Per-wave LDS of the octant-ordered pool (bounce rays only).
+template <bool kOn>
+struct SortPool {
+  uint16_t list[256 / 64][16 * 64];
+  uint32_t base[256 / 64][16];
+};
+template <>
+struct SortPool<false> {
+  uint16_t list[1][1];
+  uint32_t base[1][1];
+};
+
 template <bool kCount, bool kPrims, bool kCamera>
 __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int cursor, int lds_depth) {
   __shared__ BatchView v;
+  __shared__ SortPool<!kCamera> sp;
   const uint32_t* counts = B.counters + ctr_path(qi, 0, 0);
   load_batch(v, B, counts, false);
   if (blockIdx.x == 0)
@@ -183,8 +244,9 @@ __global__ void __launch_bounds__(256) k_closest(WaveBuffers B, int qi, int curs
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
-  trace_queue<false, kCount, kPrims>(
-      B, v, &B.counters[kCursor + cursor], 1u, st, c, rays, overflow,
+  const uint32_t wv = kCamera ? 0u : threadIdx.x >> 6;
+  trace_queue<false, kCount, kPrims, !kCamera>(
+      B, v, &B.counters[kCursor + cursor], 1u, st, c, rays, overflow, q.ray1, sp.list[wv], sp.base[wv],
       [&](uint32_t i, int ci, f3& o, f3& d, float& tmin, float& tmax) {
         if constexpr (kCamera) {
           const DevChunk& ck = v.ch[ci];
@@ -587,7 +649,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kGuide
     }
     if (want_n) {
       qn.ray0[in] = make_float4(no.x, no.y, no.z, nd.x);
-      qn.ray1[in] = make_float4(nd.y, nd.z, npdf, __uint_as_float(ndelta));
+      // flags: bit 0 delta, bits 1-3 the direction octant (the traversal
+      // groups rays by it)
+      const uint32_t oct = (nd.x < 0.f ? 1u : 0u) | (nd.y < 0.f ? 2u : 0u) | (nd.z < 0.f ? 4u : 0u);
+      qn.ray1[in] = make_float4(nd.y, nd.z, npdf, __uint_as_float(ndelta | (oct << 1)));
       qn.thr[in] = make_float4(nT.x, nT.y, nT.z, 0.f);
       qn.st[in] = make_uint4(static_cast<uint32_t>(rng.s), static_cast<uint32_t>(rng.s >> 32), slot, nrec);
     }
@@ -610,8 +675,8 @@ __global__ void __launch_bounds__(256) k_shadow(WaveBuffers B, int cursor, int l
   TravCount c{0, 0, 0, 0};
   uint32_t rays = 0;
   bool overflow = false;
-  trace_queue<true, kCount, kPrims>(
-      B, v, &B.counters[kCursor + cursor], 2u, st, c, rays, overflow,
+  trace_queue<true, kCount, kPrims, false>(
+      B, v, &B.counters[kCursor + cursor], 2u, st, c, rays, overflow, nullptr, nullptr, nullptr,
       [&](uint32_t i, int, f3& o, f3& d, float& tmin, float& tmax) {
         const float4 r0 = B.sh0[i], r1 = B.sh1[i];
         o = f3{r0.x, r0.y, r0.z};
