@@ -155,6 +155,7 @@ struct DevChunk {
   int32_t job, spp_first, nsamples, npix;
   int32_t base, cap;
   int32_t qbase, qseg;   // the chunk's path-queue segments: kQSeg of qseg entries from qbase
+  int32_t groups;        // depth-0 entry order: 1 = blocks of 8 samples x 8 pixels, 0 = sample-major
 };
 
 struct PathQueue {

@@ -56,8 +56,14 @@ __device__ __forceinline__ int tiled_pixel(int t, int W, int H) {
 }
 
 // The camera path behind depth-0 queue entry e of chunk ck: depth 0 holds a
-// chunk's paths as its segment 0 in slot order, sample-major over the tiled
-// pixels (slot = base + e - qbase). Its seed is the pixel index plus the
+// chunk's paths as its segment 0. Slots are sample-major over the tiled
+// pixels (slot = base + sample * npix + t, t the tiled pixel index). The
+// entries are in slot order, or, when ck.groups is set (8 | npix, 8 | the
+// chunk's samples), in blocks of 64: eight samples of eight consecutive
+// tiled pixels (a row of an 8x8 tile), block after block over the pixels,
+// then over the next eight samples, so that a wave's camera rays leave
+// one pixel footprint eight times smaller than a tile, while each sample's
+// eight slots stay contiguous. Its seed is the pixel index plus the
 // sample index times 0xdeadbeef, its ray the jittered pinhole ray
 // (cpu_tiled.cpp:20-29, camera/camera.h:35-37), and rng is the state after
 // the two jitter draws. Camera rays are not stored: the depth-0 traversal,
@@ -70,8 +76,18 @@ struct CameraPath {
 };
 __device__ __forceinline__ CameraPath camera_path(const DevChunk& ck, const DevJob& J, uint32_t e) {
   const int local = static_cast<int>(e) - ck.qbase;
-  const int s = local / ck.npix;
-  const int pixel = tiled_pixel(local - s * ck.npix, J.W, J.H);
+  int s, t;
+  if (ck.groups) {
+    const int blk = local >> 6, l = local & 63;
+    const int pgroups = ck.npix >> 3;
+    const int sg = blk / pgroups;
+    s = 8 * sg + (l >> 3);
+    t = 8 * (blk - sg * pgroups) + (l & 7);
+  } else {
+    s = local / ck.npix;
+    t = local - s * ck.npix;
+  }
+  const int pixel = tiled_pixel(t, J.W, J.H);
   const int spp = ck.spp_first + s;
   const uint64_t seed = static_cast<uint64_t>(pixel) + static_cast<uint64_t>(spp) * 0xdeadbeefULL;
   CameraPath p;
@@ -85,7 +101,7 @@ __device__ __forceinline__ CameraPath camera_path(const DevChunk& ck, const DevJ
   const f3 hz{J.cam.horiz[0], J.cam.horiz[1], J.cam.horiz[2]};
   const f3 vt{J.cam.vert[0], J.cam.vert[1], J.cam.vert[2]};
   p.d = normalize(llc + u * hz + w * vt - p.o);
-  p.slot = static_cast<uint32_t>(ck.base + local);
+  p.slot = static_cast<uint32_t>(ck.base + s * ck.npix + t);
   return p;
 }
 

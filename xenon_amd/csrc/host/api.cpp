@@ -821,6 +821,9 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
       ck.cap = static_cast<int32_t>(npix * ns);
       ck.qseg = static_cast<int32_t>(qseg_for(ck.cap));
       ck.qbase = static_cast<int32_t>(qused);
+      // guided batches keep slot order at depth 0 (a pixel's first sample
+      // is then entry qbase + t, where its mixture is stored)
+      ck.groups = (!c->guide.on && npix % 8 == 0 && ns % 8 == 0) ? 1 : 0;
       qused += static_cast<size_t>(kQSeg) * static_cast<size_t>(ck.qseg);
       if (qused > qlimit) return fail(c, XNH_E_INVALID, "queue segments exceed the queue capacity");
       counts.back() = ++in_batch;
