@@ -155,7 +155,8 @@ struct DevChunk {
   int32_t job, spp_first, nsamples, npix;
   int32_t base, cap;
   int32_t qbase, qseg;   // the chunk's path-queue segments: kQSeg of qseg entries from qbase
-  int32_t groups;        // depth-0 entry order: 1 = blocks of 8 samples x 8 pixels, 0 = sample-major
+  int32_t cam_lg;        // depth-0 entry order: log2 of the samples per 64-entry block (3: 8 samples of
+                         // 8 pixels, 6: 64 samples of one pixel), 0 = sample-major
 };
 
 struct PathQueue {

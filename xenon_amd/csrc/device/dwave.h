@@ -58,12 +58,13 @@ __device__ __forceinline__ int tiled_pixel(int t, int W, int H) {
 // The camera path behind depth-0 queue entry e of chunk ck: depth 0 holds a
 // chunk's paths as its segment 0. Slots are sample-major over the tiled
 // pixels (slot = base + sample * npix + t, t the tiled pixel index). The
-// entries are in slot order, or, when ck.groups is set (8 | npix, 8 | the
-// chunk's samples), in blocks of 64: eight samples of eight consecutive
-// tiled pixels (a row of an 8x8 tile), block after block over the pixels,
-// then over the next eight samples, so that a wave's camera rays leave
-// one pixel footprint eight times smaller than a tile, while each sample's
-// eight slots stay contiguous. Its seed is the pixel index plus the
+// entries are in slot order, or, when ck.cam_lg = lg is set, in blocks of
+// 64: 2^lg samples of 2^(6-lg) consecutive tiled pixels (lg 3: a row of an
+// 8x8 tile; lg 6: one pixel), block after block over the pixels, then over
+// the next 2^lg samples, so that a wave's camera rays leave a footprint
+// 2^lg times smaller than a tile, while each sample's 2^(6-lg) slots stay
+// contiguous (2^lg divides the chunk's samples, 2^(6-lg) its pixels). Its
+// seed is the pixel index plus the
 // sample index times 0xdeadbeef, its ray the jittered pinhole ray
 // (cpu_tiled.cpp:20-29, camera/camera.h:35-37), and rng is the state after
 // the two jitter draws. Camera rays are not stored: the depth-0 traversal,
@@ -77,12 +78,13 @@ struct CameraPath {
 __device__ __forceinline__ CameraPath camera_path(const DevChunk& ck, const DevJob& J, uint32_t e) {
   const int local = static_cast<int>(e) - ck.qbase;
   int s, t;
-  if (ck.groups) {
+  if (ck.cam_lg) {
+    const int lg = ck.cam_lg, lp = 6 - lg;   // log2 of the samples, pixels per block
     const int blk = local >> 6, l = local & 63;
-    const int pgroups = ck.npix >> 3;
+    const int pgroups = ck.npix >> lp;
     const int sg = blk / pgroups;
-    s = 8 * sg + (l >> 3);
-    t = 8 * (blk - sg * pgroups) + (l & 7);
+    s = (sg << lg) + (l >> lp);
+    t = ((blk - sg * pgroups) << lp) + (l & ((1 << lp) - 1));
   } else {
     s = local / ck.npix;
     t = local - s * ck.npix;

@@ -821,9 +821,21 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
       ck.cap = static_cast<int32_t>(npix * ns);
       ck.qseg = static_cast<int32_t>(qseg_for(ck.cap));
       ck.qbase = static_cast<int32_t>(qused);
-      // guided batches keep slot order at depth 0 (a pixel's first sample
-      // is then entry qbase + t, where its mixture is stored)
-      ck.groups = (!c->guide.on && npix % 8 == 0 && ns % 8 == 0) ? 1 : 0;
+      // Depth-0 order (camera_path): scenes of many instances walk their
+      // TLAS and several BLAS per camera ray and gain from the one-pixel
+      // footprint of 64 samples per block (instancing, 4096 instances:
+      // +8.7 % over 8 x 8); in smaller scenes the depth-0 per-slot writes,
+      // then 64 separate lines per wave, cost more than the coherence gains
+      // (36-triangle Cornell box -1.3 %; a 1.3M-triangle two-instance scene
+      // unchanged), and 8 samples of 8 pixels are taken. Guided batches keep
+      // slot order (a pixel's first sample is then entry qbase + t, where
+      // its mixture is stored).
+      ck.cam_lg = 0;
+      if (!c->guide.on) {
+        const int lg = js.S->num_tnodes >= 127 ? 6 : 3;   // 127 TLAS nodes: 64 instances
+        if (ns % (1 << lg) == 0 && npix % (size_t(1) << (6 - lg)) == 0) ck.cam_lg = lg;
+        else if (ns % 8 == 0 && npix % 8 == 0) ck.cam_lg = 3;
+      }
       qused += static_cast<size_t>(kQSeg) * static_cast<size_t>(ck.qseg);
       if (qused > qlimit) return fail(c, XNH_E_INVALID, "queue segments exceed the queue capacity");
       counts.back() = ++in_batch;
