@@ -28,9 +28,9 @@
 namespace xnh {
 
 // ---------------------------------------------------------------------------
-// raygen: starts a batch. Depth 0 of every chunk is its camera paths in slot
-// order (segment 0 counts the chunk's paths; the rays themselves are rebuilt
-// from the entry index by camera_path where they are read); the batch's
+// raygen: starts a batch. Depth 0 of every chunk is its camera paths
+// (segment 0 counts the chunk's paths; their order and the rays themselves
+// come from camera_path, wherever a depth-0 entry is read); the batch's
 // other queue and shadow counters, work cursors and record counters are
 // reset and every slot's radiance terms cleared. Grid-stride over the
 // batch's slots, so the grid is bounded whatever the queue capacity.
@@ -139,7 +139,11 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
               // octant (4 bits per unit and lane; 8 = no entry)
               const uint32_t n = min(uend, v.prefix[ci + 1]) - ucur;
               uint32_t keys0 = 0u, keys1 = 0u;
-              __builtin_amdgcn_wave_barrier();   // every lane has taken its entry of the last pool
+              // every lane has taken its entry of the last pool before the
+              // order is rewritten
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_wave_barrier();
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
               for (uint32_t j = 0; j < n; ++j) {
                 uint32_t first, lim;
                 const uint32_t b = unit_segment(v, ci, ucur + j - v.prefix[ci], seg, first, lim) + first;
