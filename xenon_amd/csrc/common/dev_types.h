@@ -159,11 +159,13 @@ struct DevChunk {
                          // 8 pixels, 6: 64 samples of one pixel), 0 = sample-major
 };
 
+// Bounce paths only: depth-0 (camera) paths are rebuilt from their entry
+// index and store nothing here.
 struct PathQueue {
   float4* ray0;     // {o.xyz, d.x}
-  float4* ray1;     // {d.y, d.z, prev_pdf, flags}
+  float4* ray1;     // {d.y, d.z, prev_pdf, flags: bit 0 delta, bits 1-3 direction octant}
   float4* thr;      // {T.xyz, 0}
-  uint4* st;        // {rng lo, rng hi, slot, 0}
+  uint4* st;        // {rng lo, rng hi, slot, training record or kNoRecord}
 };
 
 struct WaveBuffers {
