@@ -143,12 +143,27 @@ constexpr int64_t qseg_for(int64_t cap) { return ((cap + kQSeg - 1) / kQSeg + kS
 // needs for its segments.
 constexpr int64_t kQueueSlack = int64_t(kMaxChunks) * kQSeg * (kSegSlack + 128);
 
+// Division of n < 2^31 by a divisor d that is fixed for a batch, as one
+// 32x32->64-bit multiply and a shift: n / d = (n * m) >> p with
+// p = 31 + ceil(log2 d) and m = ceil(2^p / d) (< 2^32). m exceeds 2^p / d
+// by less than 1, so n * m / 2^p exceeds n / d by less than n / 2^p < 1/d
+// and the floor is exact (division by invariant integers using
+// multiplication, Granlund and Montgomery 1994).
+struct FastDiv {
+  uint32_t m, p;
+};
+inline FastDiv fast_div(uint32_t d) {
+  const uint32_t k = d <= 1u ? 0u : 32u - static_cast<uint32_t>(__builtin_clz(d - 1u));
+  const uint32_t p = 31u + k;
+  return FastDiv{static_cast<uint32_t>(((uint64_t(1) << p) + d - 1u) / d), p};
+}
+
 struct DevJob {
   DevScene S;
   DevCamera cam;
   int32_t W, H, minb, maxb;
   float4* accum;    // per-pixel sums of this job
-  int32_t pad[2];
+  FastDiv band;     // division by 8 W (the rows of pixel tiles)
 };
 
 struct DevChunk {
@@ -157,6 +172,7 @@ struct DevChunk {
   int32_t qbase, qseg;   // the chunk's path-queue segments: kQSeg of qseg entries from qbase
   int32_t cam_lg;        // depth-0 entry order: log2 of the samples per 64-entry block (3: 8 samples of
                          // 8 pixels, 6: 64 samples of one pixel), 0 = sample-major
+  FastDiv sdiv;          // division by npix (cam_lg 0) or by the pixel blocks npix >> (6 - cam_lg)
 };
 
 // Bounce paths only: depth-0 (camera) paths are rebuilt from their entry

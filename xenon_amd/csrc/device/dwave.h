@@ -75,26 +75,41 @@ struct CameraPath {
   Rng rng;
   uint32_t slot;
 };
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f) {
+  return static_cast<uint32_t>((static_cast<uint64_t>(n) * f.m) >> f.p);
+}
 __device__ __forceinline__ CameraPath camera_path(const DevChunk& ck, const DevJob& J, uint32_t e) {
-  const int local = static_cast<int>(e) - ck.qbase;
-  int s, t;
+  // the divisions by the chunk's and the job's sizes are multiply-shifts
+  // (FastDiv); the rest are shifts except in the last tile row and column
+  const uint32_t local = e - static_cast<uint32_t>(ck.qbase);
+  const uint32_t npix = static_cast<uint32_t>(ck.npix);
+  uint32_t s, t;
   if (ck.cam_lg) {
-    const int lg = ck.cam_lg, lp = 6 - lg;   // log2 of the samples, pixels per block
-    const int blk = local >> 6, l = local & 63;
-    const int pgroups = ck.npix >> lp;
-    const int sg = blk / pgroups;
+    const uint32_t lg = static_cast<uint32_t>(ck.cam_lg), lp = 6u - lg;   // log2 of the samples, pixels per block
+    const uint32_t blk = local >> 6, l = local & 63u;
+    const uint32_t sg = fdiv(blk, ck.sdiv);   // / (npix >> lp)
     s = (sg << lg) + (l >> lp);
-    t = ((blk - sg * pgroups) << lp) + (l & ((1 << lp) - 1));
+    t = ((blk - sg * (npix >> lp)) << lp) + (l & ((1u << lp) - 1u));
   } else {
-    s = local / ck.npix;
-    t = local - s * ck.npix;
+    s = fdiv(local, ck.sdiv);   // / npix
+    t = local - s * npix;
   }
-  const int pixel = tiled_pixel(t, J.W, J.H);
-  const int spp = ck.spp_first + s;
+  // tiled index -> pixel (x, y): the steps of tiled_pixel
+  const int W = J.W, H = J.H;
+  const int ti = static_cast<int>(t);
+  const int band = static_cast<int>(fdiv(t, J.band));   // / (8 W)
+  const int r = ti - band * 8 * W;
+  const int hb = min(8, H - 8 * band);
+  const int bx = hb == 8 ? r >> 6 : r / (8 * hb);
+  const int wb = min(8, W - 8 * bx);
+  const int q = r - bx * 8 * hb;
+  const int yi = wb == 8 ? q >> 3 : q / wb;
+  const int x = 8 * bx + (q - yi * wb), y = 8 * band + yi;
+  const int pixel = y * W + x;
+  const int spp = ck.spp_first + static_cast<int>(s);
   const uint64_t seed = static_cast<uint64_t>(pixel) + static_cast<uint64_t>(spp) * 0xdeadbeefULL;
   CameraPath p;
   p.rng = rng_for(seed);
-  const int x = pixel % J.W, y = pixel / J.W;
   const float fx = static_cast<float>(x) + p.rng.f();
   const float fy = static_cast<float>(y) + p.rng.f();
   const float u = fx / static_cast<float>(J.W), w = fy / static_cast<float>(J.H);
@@ -103,7 +118,7 @@ __device__ __forceinline__ CameraPath camera_path(const DevChunk& ck, const DevJ
   const f3 hz{J.cam.horiz[0], J.cam.horiz[1], J.cam.horiz[2]};
   const f3 vt{J.cam.vert[0], J.cam.vert[1], J.cam.vert[2]};
   p.d = normalize(llc + u * hz + w * vt - p.o);
-  p.slot = static_cast<uint32_t>(ck.base + s * ck.npix + t);
+  p.slot = static_cast<uint32_t>(ck.base) + s * npix + t;
   return p;
 }
 

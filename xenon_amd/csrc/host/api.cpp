@@ -836,6 +836,7 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
         if (ns % (1 << lg) == 0 && npix % (size_t(1) << (6 - lg)) == 0) ck.cam_lg = lg;
         else if (ns % 8 == 0 && npix % 8 == 0) ck.cam_lg = 3;
       }
+      ck.sdiv = fast_div(static_cast<uint32_t>(ck.cam_lg ? npix >> (6 - ck.cam_lg) : npix));
       qused += static_cast<size_t>(kQSeg) * static_cast<size_t>(ck.qseg);
       if (qused > qlimit) return fail(c, XNH_E_INVALID, "queue segments exceed the queue capacity");
       counts.back() = ++in_batch;
@@ -861,6 +862,7 @@ int render_jobs(xnh_ctx* c, const std::vector<JobSpec>& jobs) {
       d.cam.vert[k] = jobs[j].cam.vertical[k];
     }
     d.W = jobs[j].W;
+    d.band = fast_div(static_cast<uint32_t>(8 * jobs[j].W));
     d.H = jobs[j].H;
     d.minb = jobs[j].minb;
     d.maxb = jobs[j].maxb;
