@@ -161,3 +161,24 @@ def test_builder_matches_reference_at_full_size(reference, name):
         assert all(20_000 * 0.9 <= len(desc.meshes[m].indices) <= 200_000 * 1.1 for m in range(10))
     else:
         assert ntri > 499_000
+
+
+def test_fast_div_matches_integer_division(tmp_path):
+    """The multiply-shift division the camera-ray mapping uses (FastDiv in
+    common/dev_types.h, its multiplier computed on the host) equals integer
+    division for dividends below 2^31: tests/native/fast_div_check.cpp
+    compiled against the header itself."""
+    import shutil
+    import subprocess
+    from xenon_amd import build as xb
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ is not installed")
+    here = os.path.dirname(os.path.abspath(__file__))
+    exe = str(tmp_path / "fast_div_check")
+    subprocess.run([gxx, "-O2", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{xb.ROCM}/include", f"-I{xb.CSRC}",
+                    os.path.join(here, "native", "fast_div_check.cpp"), "-o", exe], check=True,
+                   capture_output=True, text=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "agree" in r.stdout
