@@ -100,8 +100,15 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
   // device, so a fetch per 64-entry unit capped a launch at ~5 G rays/s.
   // A fetch takes up to 16 units, fewer as the queue runs out (what is left
   // per 4 waves of the grid, as this wave last saw it), so the launch
-  // does not end with a few waves walking long private batches.
+  // does not end with a few waves walking long private batches. Batches of
+  // fewer than 64 units per wave (a 512x512 frame at 64 samples on the
+  // whole device) walk short queues at every depth: there the fetches'
+  // own cost outweighs the balance at the end of a launch, and a fetch
+  // takes up to twice as much (what is left per 2 waves).
   const uint32_t gwaves = gridDim.x * (blockDim.x >> 6);
+  const uint32_t batch_units =
+      v.nch > 0 ? static_cast<uint32_t>(v.ch[v.nch - 1].base + v.ch[v.nch - 1].cap) >> 6 : 0u;
+  const uint32_t share = batch_units < 64u * gwaves ? 2u : 4u;
   uint32_t seen = 0;              // the first unit of this wave's last fetch
   uint32_t ucur = 0, uend = 0;    // fetched units not yet taken (wave-uniform)
   bool exhausted = false;
@@ -116,7 +123,7 @@ __device__ __forceinline__ void trace_queue(const WaveBuffers& B, const BatchVie
       if (pnext >= pend) {
         if (ucur >= uend && !exhausted) {
           const uint32_t left = total > seen ? total - seen : 0u;
-          const uint32_t per = max(1u, min(16u, left / (4u * gwaves)));
+          const uint32_t per = max(1u, min(16u, left / (share * gwaves)));
           uint32_t u0 = 0;
           if (lane == 0) u0 = atomicAdd(cursor, per);
           u0 = static_cast<uint32_t>(uniform(static_cast<int>(__shfl(u0, 0))));
